@@ -1,0 +1,854 @@
+// fh_assemble.hip — numeric assembly on the device.
+//
+// Row-gather formulation of the reference's cell loops (poisson.rs:23-218,
+// elasticity.rs:73-227, mass.rs:10-82): one thread owns one block row (a
+// vertex, or a P2 dof), walks the cells incident to it in ascending cell
+// order and adds that cell's contributions for its row into the row's slots.
+// Consequences:
+//   * no atomics and no colouring: every value slot has exactly one writer;
+//   * each slot's duplicates are summed in ascending cell order, the order in
+//     which rayon's collect() lays the triplets out and the order the oracle
+//     sums them in, so FP64 values are bit-identical to the oracle;
+//   * load vectors use the same per-dof order as the reference's sequential
+//     scatter (poisson.rs:116-119, elasticity.rs:157-182), so they are
+//     bit-identical too.
+// Values are written straight into the SELL-64 layout the CG kernels read.
+#include <algorithm>
+#include <vector>
+
+#include "fh_geom.hpp"
+#include "fh_internal.hpp"
+
+namespace fh {
+
+using namespace fhg;
+
+// value index of (block row r, entry k, block component j) in SELL-64
+__device__ __forceinline__ uint64_t vidx(uint32_t soff_s, uint32_t k, int j, uint32_t lane,
+                                         int bs2) {
+    return ((uint64_t)soff_s + (uint64_t)k * SLICE) * bs2 + (uint64_t)j * SLICE + lane;
+}
+
+struct RowCtx {
+    uint32_t soff, len, lane;
+};
+__device__ __forceinline__ RowCtx row_ctx(const uint32_t* soff, const uint32_t* rlen, uint32_t r) {
+    return {soff[r / SLICE], rlen[r], r % SLICE};
+}
+
+template <int BS2>
+__device__ __forceinline__ void init_row(double* vals, const RowCtx& rc) {
+    for (uint32_t k = 0; k < rc.len; ++k)
+#pragma unroll
+        for (int j = 0; j < BS2; ++j) vals[vidx(rc.soff, k, j, rc.lane, BS2)] = -0.0;
+}
+
+// poisson.rs:33-66 (kind 0: stiffness) and mass.rs:16-41 (kind 1: mass), P1 triangles
+template <int KIND>
+__global__ __launch_bounds__(256) void k_asm_p1_tri(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
+    double* __restrict__ vals) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const RowCtx rc = row_ctx(soff, rlen, v);
+    init_row<1>(vals, rc);
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / 3, a = aent[e] % 3;
+        double x[3][2];
+        uint32_t t[3];
+        load_cell<2, 3>(V, C, c, x, t);
+        Map2 m;
+        if (!affine2(x[0], x[1], x[2], m)) continue;
+        const double area = 0.5 * fabs(m.det);
+        if (KIND == 0) {
+            double g[3][2];
+            p1_tri_grads(m, g);
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const double val = (g[a][0] * g[b][0] + g[a][1] * g[b][1]) * area;
+                vals[vidx(rc.soff, slots[(uint64_t)e * 3 + b], 0, rc.lane, 1)] += val;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const double factor = ((int)a == b) ? 1.0 / 6.0 : 1.0 / 12.0;
+                vals[vidx(rc.soff, slots[(uint64_t)e * 3 + b], 0, rc.lane, 1)] += area * factor;
+            }
+        }
+    }
+}
+
+// poisson.rs:129-162 (kind 0) and mass.rs:53-79 (kind 1), P1 tetrahedra
+template <int KIND>
+__global__ __launch_bounds__(256) void k_asm_p1_tet(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
+    double* __restrict__ vals) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const RowCtx rc = row_ctx(soff, rlen, v);
+    init_row<1>(vals, rc);
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / 4, a = aent[e] % 4;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double vol = (1.0 / 6.0) * fabs(m.det);
+        if (KIND == 0) {
+            double g[4][3];
+            p1_tet_grads(m, g);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double val = (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]) * vol;
+                vals[vidx(rc.soff, slots[(uint64_t)e * 4 + b], 0, rc.lane, 1)] += val;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double factor = ((int)a == b) ? 1.0 / 10.0 : 1.0 / 20.0;
+                vals[vidx(rc.soff, slots[(uint64_t)e * 4 + b], 0, rc.lane, 1)] += vol * factor;
+            }
+        }
+    }
+}
+
+// elasticity.rs:86-146: the 3x3 blocks of row vertex v (local index a) against
+// every vertex b of each incident cell; E per cell when Ecell != nullptr.
+__global__ __launch_bounds__(256) void k_asm_elasticity(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv, double E,
+    double nu, const double* __restrict__ Ecell, double* __restrict__ vals) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const RowCtx rc = row_ctx(soff, rlen, v);
+    init_row<9>(vals, rc);
+    double lambda = lame_lambda(E, nu), mu = lame_mu(E, nu);
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / 4, a = aent[e] % 4;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        if (Ecell) {
+            const double ec = Ecell[c];
+            lambda = lame_lambda(ec, nu);
+            mu = lame_mu(ec, nu);
+        }
+        const double vol = (1.0 / 6.0) * fabs(m.det);
+        double g[4][3];
+        p1_tet_grads(m, g);
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t k = slots[(uint64_t)e * 4 + b];
+#pragma unroll
+            for (int al = 0; al < 3; ++al)
+#pragma unroll
+                for (int be = 0; be < 3; ++be)
+                    vals[vidx(rc.soff, k, al * 3 + be, rc.lane, 9)] +=
+                        elas_entry(g[a], g[b], al, be, lambda, mu, vol);
+        }
+    }
+}
+
+// P2 tetrahedron stiffness, FP64 VALU: K(a,b) = sum_q (w_q |det J|) (g_a^q . g_b^q)
+// with g^q = J^{-T} grad phi(xi_q) (lagrange.rs:324-346 at the
+// quadrature.rs:157-184 points).  G = reference gradients [q][dof][3].
+struct P2Tab {
+    double G[4][10][3];
+    double w[4];
+};
+__global__ __launch_bounds__(256) void k_asm_p2_valu(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nd, P2Tab tab,
+    double* __restrict__ vals) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    const RowCtx rc = row_ctx(soff, rlen, d);
+    init_row<1>(vals, rc);
+    for (uint32_t e = aptr[d]; e < aptr[d + 1]; ++e) {
+        const uint32_t c = aent[e] / 10, a = aent[e] % 10;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double detj = fabs(m.det);
+        double ga[4][3], s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s[q] = tab.w[q] * detj;
+            grad3(m, tab.G[q][a], ga[q]);
+        }
+        for (int b = 0; b < 10; ++b) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double gb[3];
+                grad3(m, tab.G[q][b], gb);
+                acc += s[q] * (ga[q][0] * gb[0] + ga[q][1] * gb[1] + ga[q][2] * gb[2]);
+            }
+            vals[vidx(rc.soff, slots[(uint64_t)e * 10 + b], 0, rc.lane, 1)] += acc;
+        }
+    }
+}
+
+// poisson.rs:80-119 (2D, K=3) / 176-216 (3D, K=4): b_v = sum over incident cells
+// (ascending) of sum_q (w_q |det J|) f(x_q) phi_a(xi_q).  phi = basis at points.
+template <int D>
+__global__ __launch_bounds__(256) void k_rhs_p1(const double* __restrict__ V,
+                                                const uint32_t* __restrict__ C,
+                                                const uint32_t* __restrict__ aptr,
+                                                const uint32_t* __restrict__ aent, uint32_t nv,
+                                                const double* __restrict__ fq, int nq,
+                                                const double* __restrict__ wq,
+                                                const double* __restrict__ phi,
+                                                double* __restrict__ b) {
+    constexpr int K = D + 1;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double acc = 0.0;
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / K, a = aent[e] % K;
+        double x[K][D];
+        uint32_t t[K];
+        load_cell<D, K>(V, C, c, x, t);
+        double det;
+        if constexpr (D == 2) {
+            Map2 m;
+            if (!affine2(x[0], x[1], x[2], m)) continue;
+            det = m.det;
+        } else {
+            Map3 m;
+            if (!affine3(x[0], x[1], x[2], x[K - 1], m)) continue;
+            det = m.det;
+        }
+        const double detj = fabs(det);
+        double lb = 0.0;
+        for (int q = 0; q < nq; ++q) lb += wq[q] * detj * fq[(uint64_t)c * nq + q] * phi[q * K + a];
+        acc += lb;
+    }
+    b[v] = acc;
+}
+
+// P2 load with a per-cell constant source: the same formula with 10 basis functions.
+__global__ __launch_bounds__(256) void k_rhs_p2(const double* __restrict__ V,
+                                                const uint32_t* __restrict__ C,
+                                                const uint32_t* __restrict__ aptr,
+                                                const uint32_t* __restrict__ aent, uint32_t nd,
+                                                const double* __restrict__ fcell, P2Tab tab,
+                                                const double* __restrict__ phi,
+                                                double* __restrict__ b) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    double acc = 0.0;
+    for (uint32_t e = aptr[d]; e < aptr[d + 1]; ++e) {
+        const uint32_t c = aent[e] / 10, a = aent[e] % 10;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double detj = fabs(m.det);
+        double lb = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lb += tab.w[q] * detj * fcell[c] * phi[q * 10 + a];
+        acc += lb;
+    }
+    b[d] = acc;
+}
+
+// elasticity.rs:157-182: b(3v+alpha) = sum over incident cells of f_alpha (vol/4).
+__global__ __launch_bounds__(256) void k_body_force(const double* __restrict__ V,
+                                                    const uint32_t* __restrict__ C,
+                                                    const uint32_t* __restrict__ aptr,
+                                                    const uint32_t* __restrict__ aent, uint32_t nv,
+                                                    double f0, double f1, double f2,
+                                                    double* __restrict__ b) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / 4;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double vol = (1.0 / 6.0) * fabs(m.det);
+        acc0 += f0 * (vol * 0.25);
+        acc1 += f1 * (vol * 0.25);
+        acc2 += f2 * (vol * 0.25);
+    }
+    b[3 * (uint64_t)v] = acc0;
+    b[3 * (uint64_t)v + 1] = acc1;
+    b[3 * (uint64_t)v + 2] = acc2;
+}
+
+// ---------------------------------------------------------------- Dirichlet
+// poisson.rs:231-234 / elasticity.rs:197-200: the last listing of a dof
+// decides its value; every listing appends one (d, d, 1.0) triplet.
+__global__ void k_bc_mark(const uint32_t* __restrict__ dofs, uint64_t nfix,
+                          uint32_t* __restrict__ winner, uint32_t* __restrict__ count) {
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= nfix) return;
+    atomicMax(&winner[dofs[k]], (uint32_t)(k + 1));
+    atomicAdd(&count[dofs[k]], 1u);
+}
+__global__ void k_bc_values(const uint32_t* __restrict__ winner, const double* __restrict__ vals,
+                            uint32_t n, uint8_t* __restrict__ fixed, double* __restrict__ g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = winner[i];
+    fixed[i] = w ? 1 : 0;
+    g[i] = w ? vals[w - 1] : 0.0;
+}
+
+// poisson.rs:242-259 row by row: for a free row r, b_r -= A_rc g_c over the
+// fixed columns c in ascending order and those couplings become explicit
+// zeros; a fixed row becomes (count) on the diagonal and b_d = g_d.
+template <int BS>
+__global__ __launch_bounds__(256) void k_condense(const uint32_t* __restrict__ soff,
+                                                  const uint32_t* __restrict__ rlen,
+                                                  const uint32_t* __restrict__ scols,
+                                                  uint32_t nrows, const uint8_t* __restrict__ fixed,
+                                                  const double* __restrict__ g,
+                                                  const uint32_t* __restrict__ count,
+                                                  const double* __restrict__ b_in,
+                                                  double* __restrict__ vals,
+                                                  double* __restrict__ b_out) {
+    constexpr int BS2 = BS * BS;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)nrows * BS) return;
+    const uint32_t r = t / BS, al = t % BS;
+    const uint64_t i = t;  // scalar row
+    const RowCtx rc = row_ctx(soff, rlen, r);
+    if (fixed[i]) {
+        for (uint32_t k = 0; k < rc.len; ++k) {
+            const uint32_t cb = scols[(uint64_t)rc.soff + (uint64_t)k * SLICE + rc.lane];
+#pragma unroll
+            for (int be = 0; be < BS; ++be) {
+                const uint64_t col = (uint64_t)cb * BS + be;
+                vals[vidx(rc.soff, k, al * BS + be, rc.lane, BS2)] =
+                    (col == i) ? (double)count[i] : 0.0;
+            }
+        }
+        b_out[i] = g[i];
+        return;
+    }
+    double acc = b_in[i];
+    for (uint32_t k = 0; k < rc.len; ++k) {
+        const uint32_t cb = scols[(uint64_t)rc.soff + (uint64_t)k * SLICE + rc.lane];
+#pragma unroll
+        for (int be = 0; be < BS; ++be) {
+            const uint64_t col = (uint64_t)cb * BS + be;
+            if (fixed[col]) {
+                const uint64_t ix = vidx(rc.soff, k, al * BS + be, rc.lane, BS2);
+                acc -= vals[ix] * g[col];
+                vals[ix] = 0.0;
+            }
+        }
+    }
+    b_out[i] = acc;
+}
+
+template <class T>
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------- host side
+
+static uint64_t scalar_nnz(const Pattern& p, int bs) { return p.nnzb * (uint64_t)bs * bs; }
+
+uint64_t sell_value_count(const fh_matrix* a) {
+    return a->pat->sell_entries * (uint64_t)a->bs * a->bs;
+}
+
+void ensure_vals32(fh_matrix* a) {
+    if (a->vals32.p) return;
+    DeviceCtx& d = device(a->dev);
+    const uint64_t n = sell_value_count(a);
+    a->vals32.alloc(a->dev, std::max<uint64_t>(n, 1));
+    const unsigned g = std::min<uint64_t>(cdiv(n, 256), 16384);
+    hipLaunchKernelGGL(k_to_f32<float>, dim3(std::max(g, 1u)), dim3(256), 0, d.stream, a->vals.p,
+                       a->vals32.p, n);
+    FH_LAUNCH_CHECK("to_f32");
+}
+
+static fh_matrix* new_matrix(int dev, std::shared_ptr<Pattern> pat, int bs) {
+    auto* a = new fh_matrix;
+    a->dev = dev;
+    a->bs = bs;
+    a->pat = std::move(pat);
+    a->n = a->pat->nrows * bs;
+    a->nnz_scalar = scalar_nnz(*a->pat, bs);
+    a->vals.alloc(dev, std::max<uint64_t>(a->pat->sell_entries * bs * bs, 1));
+    return a;
+}
+
+static P2Tab p2_table() {
+    // lagrange.rs:324-346 gradients at the degree-2 points (host arithmetic,
+    // identical to the oracle's table)
+    P2Tab t{};
+    const double s5 = std::sqrt(5.0);
+    const double a = (5.0 - s5) / 20.0, b = (5.0 + 3.0 * s5) / 20.0;
+    const double P[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+    for (int q = 0; q < 4; ++q) {
+        const double x = P[q][0], y = P[q][1], z = P[q][2];
+        const double l0 = 1.0 - x - y - z, l1 = x, l2 = y, l3 = z;
+        const double a0 = -(4.0 * l0 - 1.0);
+        const double G[10][3] = {{a0, a0, a0},
+                                 {4.0 * l1 - 1.0, 0.0, 0.0},
+                                 {0.0, 4.0 * l2 - 1.0, 0.0},
+                                 {0.0, 0.0, 4.0 * l3 - 1.0},
+                                 {4.0 * (l0 - l1), -4.0 * l1, -4.0 * l1},
+                                 {4.0 * l2, 4.0 * l1, 0.0},
+                                 {-4.0 * l2, 4.0 * (l0 - l2), -4.0 * l2},
+                                 {-4.0 * l3, -4.0 * l3, 4.0 * (l0 - l3)},
+                                 {4.0 * l3, 0.0, 4.0 * l1},
+                                 {0.0, 4.0 * l3, 4.0 * l2}};
+        for (int i = 0; i < 10; ++i)
+            for (int k = 0; k < 3; ++k) t.G[q][i][k] = G[i][k];
+        t.w[q] = 1.0 / 24.0;
+    }
+    return t;
+}
+
+// P2 basis values at the degree-2 points, [q][10] (lagrange.rs:301-320)
+static std::vector<double> p2_basis_table() {
+    const double s5 = std::sqrt(5.0);
+    const double a = (5.0 - s5) / 20.0, b = (5.0 + 3.0 * s5) / 20.0;
+    const double P[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+    std::vector<double> phi(40);
+    for (int q = 0; q < 4; ++q) {
+        const double x = P[q][0], y = P[q][1], z = P[q][2];
+        const double l0 = 1.0 - x - y - z, l1 = x, l2 = y, l3 = z;
+        const double v[10] = {l0 * (2.0 * l0 - 1.0), l1 * (2.0 * l1 - 1.0), l2 * (2.0 * l2 - 1.0),
+                              l3 * (2.0 * l3 - 1.0), 4.0 * l0 * l1, 4.0 * l1 * l2,
+                              4.0 * l0 * l2, 4.0 * l0 * l3, 4.0 * l1 * l3, 4.0 * l2 * l3};
+        for (int i = 0; i < 10; ++i) phi[q * 10 + i] = v[i];
+    }
+    return phi;
+}
+
+// P1 basis values [q][K] at the degree-2 points and the weights.
+struct P1Tab {
+    std::vector<double> w, phi;
+    int nq;
+};
+static P1Tab p1_table(int dim) {
+    const QuadTab q = quad_table(dim, 2);
+    P1Tab t;
+    t.nq = q.n;
+    const int K = dim + 1;
+    for (int i = 0; i < q.n; ++i) {
+        t.w.push_back(q.w[i]);
+        const double* x = q.pts[i];
+        if (dim == 2) {
+            t.phi.push_back(1.0 - x[0] - x[1]);
+            t.phi.push_back(x[0]);
+            t.phi.push_back(x[1]);
+        } else {
+            t.phi.push_back(1.0 - x[0] - x[1] - x[2]);
+            t.phi.push_back(x[0]);
+            t.phi.push_back(x[1]);
+            t.phi.push_back(x[2]);
+        }
+        (void)K;
+    }
+    return t;
+}
+
+void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ecell_dev,
+                              fh_matrix* a, hipStream_t s) {
+    Adjacency& adj = mesh_adjacency(m);
+    const Pattern& p = *a->pat;
+    hipLaunchKernelGGL(k_asm_elasticity, dim3(cdiv(m->nv, 256)), dim3(256), 0, s, m->verts.p,
+                       m->cells.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
+                       m->nv, E, nu, Ecell_dev, a->vals.p);
+    FH_LAUNCH_CHECK("asm_elasticity");
+}
+
+fh_matrix* assemble_elasticity(fh_mesh* m, double E, double nu, const double* Ecell_dev) {
+    FH_CHECK(m->dim == 3, "elasticity needs a tetrahedral mesh");
+    FH_CHECK(nu > -1.0 && nu < 0.5, "Poisson's ratio must be in (-1.0, 0.5) for physical stability");
+    DeviceCtx& d = device(m->dev);
+    auto pat = mesh_pattern(m);
+    std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 3));
+    assemble_elasticity_into(m, E, nu, Ecell_dev, a.get(), d.stream);
+    return a.release();
+}
+
+// Condense a copy of `a` (values shared pattern) and lift b (device).
+fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
+                           const uint64_t* dofs, const double* vals, double* b_out,
+                           hipStream_t s) {
+    const int dev = a->dev;
+    for (uint64_t k = 0; k < nfix; ++k)
+        FH_CHECK(dofs[k] < a->n, "Dirichlet dof out of range");
+    std::vector<uint32_t> d32(nfix);
+    for (uint64_t k = 0; k < nfix; ++k) d32[k] = (uint32_t)dofs[k];
+    DBuf<uint32_t> ddofs(dev, std::max<uint64_t>(nfix, 1)), winner(dev, a->n);
+    DBuf<double> dvals(dev, std::max<uint64_t>(nfix, 1)), g(dev, a->n);
+    h2d(ddofs.p, d32.data(), nfix, s);
+    h2d(dvals.p, vals, nfix, s);
+    std::unique_ptr<fh_matrix> o(new fh_matrix);
+    o->dev = dev;
+    o->bs = a->bs;
+    o->pat = a->pat;
+    o->n = a->n;
+    o->nnz_scalar = 0;
+    o->condensed = true;
+    o->vals.alloc(dev, a->vals.n);
+    d2d(o->vals.p, a->vals.p, a->vals.n, s);
+    o->fixed.alloc(dev, a->n);
+    o->fixed_count.alloc(dev, a->n);
+    fill_u32(winner.p, a->n, 0, s);
+    fill_u32(o->fixed_count.p, a->n, 0, s);
+    if (nfix)
+        hipLaunchKernelGGL(k_bc_mark, dim3(cdiv(nfix, 256)), dim3(256), 0, s, ddofs.p, nfix,
+                           winner.p, o->fixed_count.p);
+    hipLaunchKernelGGL(k_bc_values, dim3(cdiv(a->n, 256)), dim3(256), 0, s, winner.p, dvals.p,
+                       a->n, o->fixed.p, g.p);
+    const Pattern& p = *a->pat;
+    const uint64_t nt = (uint64_t)p.nrows * a->bs;
+    if (a->bs == 1)
+        hipLaunchKernelGGL(k_condense<1>, dim3(cdiv(nt, 256)), dim3(256), 0, s, p.slice_off.p,
+                           p.row_len.p, p.sell_cols.p, p.nrows, o->fixed.p, g.p,
+                           o->fixed_count.p, b_in, o->vals.p, b_out);
+    else
+        hipLaunchKernelGGL(k_condense<3>, dim3(cdiv(nt, 256)), dim3(256), 0, s, p.slice_off.p,
+                           p.row_len.p, p.sell_cols.p, p.nrows, o->fixed.p, g.p,
+                           o->fixed_count.p, b_in, o->vals.p, b_out);
+    FH_LAUNCH_CHECK("condense");
+    FH_HIP(hipStreamSynchronize(s));
+    // scalar nnz of the condensed CSC: both-free couplings + one per fixed dof
+    return o.release();
+}
+
+// Host export to faer's canonical CSC raw parts (optionally dropping the
+// condensed couplings).  Not on the hot path.
+struct HostCsr {
+    std::vector<uint64_t> rp;
+    std::vector<uint32_t> ci;
+    std::vector<double> v;
+};
+static HostCsr export_scalar_csr(const fh_matrix* a) {
+    DeviceCtx& d = device(a->dev);
+    const Pattern& p = *a->pat;
+    const int bs = a->bs, bs2 = bs * bs;
+    std::vector<uint32_t> rptr(p.nrows + 1), cols(p.nnzb), soff(p.nslices + 1);
+    std::vector<double> vals(sell_value_count(a));
+    std::vector<uint8_t> fixed;
+    std::vector<uint32_t> cnt;
+    d2h(rptr.data(), p.row_ptr.p, rptr.size(), d.stream);
+    d2h(cols.data(), p.cols.p, cols.size(), d.stream);
+    d2h(soff.data(), p.slice_off.p, soff.size(), d.stream);
+    d2h(vals.data(), a->vals.p, vals.size(), d.stream);
+    if (a->condensed) {
+        fixed.resize(a->n);
+        d2h(fixed.data(), a->fixed.p, a->n, d.stream);
+    }
+    FH_HIP(hipStreamSynchronize(d.stream));
+    HostCsr h;
+    h.rp.assign((size_t)a->n + 1, 0);
+    h.ci.reserve(p.nnzb * bs2);
+    h.v.reserve(p.nnzb * bs2);
+    for (uint32_t r = 0; r < p.nrows; ++r) {
+        const uint32_t s = r / SLICE, l = r % SLICE, len = rptr[r + 1] - rptr[r];
+        for (int al = 0; al < bs; ++al) {
+            const uint64_t i = (uint64_t)r * bs + al;
+            for (uint32_t k = 0; k < len; ++k) {
+                const uint32_t cb = cols[rptr[r] + k];
+                for (int be = 0; be < bs; ++be) {
+                    const uint64_t c = (uint64_t)cb * bs + be;
+                    if (a->condensed && (fixed[i] || fixed[c]) && i != c) continue;
+                    const uint64_t ix = ((uint64_t)soff[s] + (uint64_t)k * SLICE) * bs2 +
+                                        (uint64_t)(al * bs + be) * SLICE + l;
+                    h.ci.push_back((uint32_t)c);
+                    h.v.push_back(vals[ix]);
+                }
+            }
+            h.rp[i + 1] = h.ci.size();
+        }
+    }
+    return h;
+}
+
+// kind 0: stiffness (poisson.rs:23-69 / 124-165), kind 1: mass (mass.rs:10-82)
+fh_matrix* assemble_p1(fh_mesh* m, int kind) {
+    hipStream_t s = device(m->dev).stream;
+    auto pat = mesh_pattern(m);
+    Adjacency& adj = mesh_adjacency(m);
+    std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 1));
+    const dim3 g(cdiv(m->nv, 256)), b(256);
+    if (m->dim == 2) {
+        if (kind == 0)
+            hipLaunchKernelGGL(k_asm_p1_tri<0>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
+                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
+                               a->vals.p);
+        else
+            hipLaunchKernelGGL(k_asm_p1_tri<1>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
+                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
+                               a->vals.p);
+    } else {
+        if (kind == 0)
+            hipLaunchKernelGGL(k_asm_p1_tet<0>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
+                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
+                               a->vals.p);
+        else
+            hipLaunchKernelGGL(k_asm_p1_tet<1>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
+                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
+                               a->vals.p);
+    }
+    FH_LAUNCH_CHECK("asm_p1");
+    return a.release();
+}
+
+// f sampled at the degree-2 points (device, [nc][nq]) -> b (device, [nv])
+void rhs_p1_dev(fh_mesh* m, const double* fq, double* b, hipStream_t s) {
+    Adjacency& adj = mesh_adjacency(m);
+    const P1Tab t = p1_table(m->dim);
+    DBuf<double> dw(m->dev, t.w.size()), dphi(m->dev, t.phi.size());
+    h2d(dw.p, t.w.data(), t.w.size(), s);
+    h2d(dphi.p, t.phi.data(), t.phi.size(), s);
+    if (m->dim == 2)
+        hipLaunchKernelGGL(k_rhs_p1<2>, dim3(cdiv(m->nv, 256)), dim3(256), 0, s, m->verts.p,
+                           m->cells.p, adj.ptr.p, adj.ent.p, m->nv, fq, t.nq, dw.p, dphi.p, b);
+    else
+        hipLaunchKernelGGL(k_rhs_p1<3>, dim3(cdiv(m->nv, 256)), dim3(256), 0, s, m->verts.p,
+                           m->cells.p, adj.ptr.p, adj.ent.p, m->nv, fq, t.nq, dw.p, dphi.p, b);
+    FH_LAUNCH_CHECK("rhs_p1");
+    FH_HIP(hipStreamSynchronize(s));  // the small tables above are freed on return
+}
+
+void body_force_dev(fh_mesh* m, const double f[3], double* b, hipStream_t s) {
+    FH_CHECK(m->dim == 3, "body force needs a tetrahedral mesh");
+    Adjacency& adj = mesh_adjacency(m);
+    hipLaunchKernelGGL(k_body_force, dim3(cdiv(m->nv, 256)), dim3(256), 0, s, m->verts.p,
+                       m->cells.p, adj.ptr.p, adj.ent.p, m->nv, f[0], f[1], f[2], b);
+    FH_LAUNCH_CHECK("body_force");
+}
+
+}  // namespace fh
+
+using namespace fh;
+
+static hipStream_t stream_of(int dev) { return device(dev).stream; }
+
+extern "C" {
+
+int fh_assemble_stiffness(fh_mesh* m, fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        *out = assemble_p1(m, 0);
+        FH_HIP(hipStreamSynchronize(stream_of(m->dev)));
+    });
+}
+
+int fh_assemble_mass(fh_mesh* m, fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        *out = assemble_p1(m, 1);
+        FH_HIP(hipStreamSynchronize(stream_of(m->dev)));
+    });
+}
+
+int fh_assemble_elasticity_dev(fh_mesh* m, double E, double nu, const double* E_cell_dev,
+                               fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        *out = assemble_elasticity(m, E, nu, E_cell_dev);
+        FH_HIP(hipStreamSynchronize(stream_of(m->dev)));
+    });
+}
+
+int fh_assemble_elasticity(fh_mesh* m, double E, double nu, const double* E_cell,
+                           fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        DBuf<double> ec;
+        if (E_cell) {
+            ec.alloc(m->dev, m->nc);
+            h2d(ec.p, E_cell, m->nc, s);
+        }
+        *out = assemble_elasticity(m, E, nu, ec.p);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_assemble_rhs(fh_mesh* m, const double* fq, double* b_out) {
+    return guard([&] {
+        FH_CHECK(m && fq && b_out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        const int nq = m->dim == 2 ? 3 : 4;
+        DBuf<double> dfq(m->dev, (size_t)m->nc * nq), db(m->dev, m->nv);
+        h2d(dfq.p, fq, dfq.n, s);
+        rhs_p1_dev(m, dfq.p, db.p, s);
+        d2h(b_out, db.p, m->nv, s);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_assemble_body_force_dev(fh_mesh* m, const double f[3], double* b_dev) {
+    return guard([&] {
+        FH_CHECK(m && f && b_dev, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        body_force_dev(m, f, b_dev, s);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_assemble_body_force(fh_mesh* m, const double f[3], double* b_out) {
+    return guard([&] {
+        FH_CHECK(m && f && b_out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        DBuf<double> db(m->dev, 3 * (size_t)m->nv);
+        if (fh_assemble_body_force_dev(m, f, db.p) != FH_OK) throw Error(FH_ERR_HIP, fh_last_error());
+        d2h(b_out, db.p, db.n, s);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        MeshP2& p2 = mesh_p2(m);
+        std::unique_ptr<fh_matrix> a(new_matrix(m->dev, p2.pat, 1));
+        if (method == 0) {
+            hipLaunchKernelGGL(k_asm_p2_valu, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s,
+                               m->verts.p, m->cells.p, p2.adj->ptr.p, p2.adj->ent.p,
+                               p2.adj->slots.p, p2.pat->slice_off.p, p2.pat->row_len.p,
+                               p2.adj->ndofs, p2_table(), a->vals.p);
+            FH_LAUNCH_CHECK("asm_p2_valu");
+        } else {
+            throw Error(FH_ERR_UNSUPPORTED, "P2 MFMA assembly not built");
+        }
+        FH_HIP(hipStreamSynchronize(s));
+        *out = a.release();
+    });
+}
+
+int fh_p2_info(fh_mesh* m, uint64_t* ndofs, uint64_t* nedges) {
+    return guard([&] {
+        FH_CHECK(m, "null mesh");
+        MeshP2& p2 = mesh_p2(m);
+        if (ndofs) *ndofs = p2.adj->ndofs;
+        if (nedges) *nedges = p2.nedges;
+    });
+}
+
+int fh_p2_cell_dofs(fh_mesh* m, uint64_t* out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        MeshP2& p2 = mesh_p2(m);
+        std::vector<uint32_t> h((size_t)m->nc * 10);
+        d2h(h.data(), p2.cell_dofs.p, h.size(), s);
+        FH_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+    });
+}
+
+int fh_assemble_p2_rhs_cellconst(fh_mesh* m, const double* f_cell, double* b_out) {
+    return guard([&] {
+        FH_CHECK(m && f_cell && b_out, "null argument");
+        hipStream_t s = stream_of(m->dev);
+        MeshP2& p2 = mesh_p2(m);
+        const std::vector<double> phi = p2_basis_table();
+        DBuf<double> df(m->dev, m->nc), dphi(m->dev, 40), db(m->dev, p2.adj->ndofs);
+        h2d(df.p, f_cell, m->nc, s);
+        h2d(dphi.p, phi.data(), 40, s);
+        hipLaunchKernelGGL(k_rhs_p2, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s, m->verts.p,
+                           m->cells.p, p2.adj->ptr.p, p2.adj->ent.p, p2.adj->ndofs, df.p,
+                           p2_table(), dphi.p, db.p);
+        FH_LAUNCH_CHECK("rhs_p2");
+        d2h(b_out, db.p, db.n, s);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_apply_dirichlet_dev(const fh_matrix* a, const double* b_in_dev, uint64_t nfix,
+                           const uint64_t* dofs, const double* vals, fh_matrix** out,
+                           double* b_out_dev) {
+    return guard([&] {
+        FH_CHECK(a && b_in_dev && out && b_out_dev, "null argument");
+        FH_CHECK(nfix == 0 || (dofs && vals), "null Dirichlet list");
+        *out = apply_dirichlet(a, b_in_dev, nfix, dofs, vals, b_out_dev, stream_of(a->dev));
+    });
+}
+
+int fh_apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
+                       const uint64_t* dofs, const double* vals, fh_matrix** out,
+                       double* b_out) {
+    return guard([&] {
+        FH_CHECK(a && b_in && out && b_out, "null argument");
+        FH_CHECK(nfix == 0 || (dofs && vals), "null Dirichlet list");
+        hipStream_t s = stream_of(a->dev);
+        DBuf<double> bi(a->dev, a->n), bo(a->dev, a->n);
+        h2d(bi.p, b_in, a->n, s);
+        *out = apply_dirichlet(a, bi.p, nfix, dofs, vals, bo.p, s);
+        d2h(b_out, bo.p, a->n, s);
+        FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_matrix_info(const fh_matrix* a, uint64_t* n, uint64_t* nnz, int* block) {
+    return guard([&] {
+        FH_CHECK(a, "null matrix");
+        if (n) *n = a->n;
+        if (block) *block = a->bs;
+        if (nnz) {
+            if (!a->condensed) {
+                *nnz = a->nnz_scalar;
+            } else {
+                const HostCsr h = export_scalar_csr(a);
+                *nnz = h.ci.size();
+                const_cast<fh_matrix*>(a)->nnz_scalar = h.ci.size();
+            }
+        }
+    });
+}
+
+int fh_matrix_export_csc(const fh_matrix* a, uint64_t* col_ptr, uint64_t* row_idx, double* vals) {
+    return guard([&] {
+        FH_CHECK(a && col_ptr, "null argument");
+        const HostCsr h = export_scalar_csr(a);
+        const uint64_t n = a->n, nnz = h.ci.size();
+        // transpose the scalar CSR (rows ascending within each column)
+        std::vector<uint64_t> cp(n + 1, 0);
+        for (uint64_t i = 0; i < nnz; ++i) cp[h.ci[i] + 1]++;
+        for (uint64_t c = 0; c < n; ++c) cp[c + 1] += cp[c];
+        std::copy(cp.begin(), cp.end(), col_ptr);
+        if (row_idx || vals) {
+            std::vector<uint64_t> cur(cp.begin(), cp.end() - 1);
+            for (uint64_t r = 0; r < n; ++r)
+                for (uint64_t p = h.rp[r]; p < h.rp[r + 1]; ++p) {
+                    const uint64_t q = cur[h.ci[p]]++;
+                    if (row_idx) row_idx[q] = r;
+                    if (vals) vals[q] = h.v[p];
+                }
+        }
+    });
+}
+
+int fh_matrix_destroy(fh_matrix* a) {
+    return guard([&] {
+        if (!a) return;
+        (void)hipSetDevice(a->dev);
+        delete a;
+    });
+}
+
+}  // extern "C"

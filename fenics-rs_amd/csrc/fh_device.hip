@@ -1,0 +1,244 @@
+// fh_device.hip — device discovery (replaces the dlopen'd CUDA driver table of
+// fenics-rs gpu.rs:36-242), per-device contexts, the caching allocator, fill
+// kernels and a deterministic prefix sum.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "fh_internal.hpp"
+
+namespace fh {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+static std::mutex g_dev_mu;
+static std::map<int, std::unique_ptr<DeviceCtx>> g_devs;
+
+int resolve_device(int dev) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        throw Error(FH_ERR_NO_DEVICE, "no HIP device available");
+    if (dev >= 0) {
+        if (dev >= count) throw Error(FH_ERR_BAD_INPUT, "device index out of range");
+        return dev;
+    }
+    // gpu.rs:237-242: the device with the largest free memory
+    int best = 0;
+    size_t best_free = 0;
+    for (int i = 0; i < count; ++i) {
+        FH_HIP(hipSetDevice(i));
+        size_t fr = 0, tot = 0;
+        FH_HIP(hipMemGetInfo(&fr, &tot));
+        if (fr > best_free) {
+            best_free = fr;
+            best = i;
+        }
+    }
+    return best;
+}
+
+DeviceCtx& device(int dev) {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto it = g_devs.find(dev);
+    if (it == g_devs.end()) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            throw Error(FH_ERR_NO_DEVICE, "no HIP device available");
+        if (dev < 0 || dev >= count) throw Error(FH_ERR_BAD_INPUT, "device index out of range");
+        auto c = std::make_unique<DeviceCtx>();
+        c->id = dev;
+        FH_HIP(hipSetDevice(dev));
+        FH_HIP(hipGetDeviceProperties(&c->prop, dev));
+        c->n_cu = c->prop.multiProcessorCount;
+        FH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        it = g_devs.emplace(dev, std::move(c)).first;
+    }
+    FH_HIP(hipSetDevice(dev));
+    return *it->second;
+}
+
+// ---------------------------------------------------------------- allocator
+static std::mutex g_mem_mu;
+static std::map<int, std::multimap<size_t, void*>> g_free;
+
+void* dmalloc(int dev, size_t bytes) {
+    if (!bytes) return nullptr;
+    bytes = (bytes + 255) & ~(size_t)255;
+    {
+        std::lock_guard<std::mutex> lk(g_mem_mu);
+        auto& fl = g_free[dev];
+        auto it = fl.find(bytes);
+        if (it != fl.end()) {
+            void* p = it->second;
+            fl.erase(it);
+            return p;
+        }
+    }
+    FH_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        release_cache(dev);
+        FH_HIP(hipSetDevice(dev));
+        FH_HIP(hipMalloc(&p, bytes));
+    }
+    return p;
+}
+
+void dfree(int dev, void* p, size_t bytes) {
+    if (!p) return;
+    bytes = (bytes + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_free[dev].emplace(bytes, p);
+}
+
+void release_cache(int dev) {
+    std::multimap<size_t, void*> fl;
+    {
+        std::lock_guard<std::mutex> lk(g_mem_mu);
+        fl.swap(g_free[dev]);
+    }
+    if (fl.empty()) return;
+    // frees must not race with kernels still using recycled blocks
+    (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : fl) (void)hipFree(kv.second);
+}
+
+// ---------------------------------------------------------------- fills
+template <class T>
+__global__ void k_fill(T* __restrict__ p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+template <class T>
+static void fill(T* p, size_t n, T v, hipStream_t s) {
+    if (!n) return;
+    const unsigned g = std::min<size_t>(cdiv(n, 256), 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3(g), dim3(256), 0, s, p, n, v);
+    FH_LAUNCH_CHECK("fill");
+}
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) { fill(p, n, v, s); }
+void fill_f64(double* p, size_t n, double v, hipStream_t s) { fill(p, n, v, s); }
+void fill_f32(float* p, size_t n, float v, hipStream_t s) { fill(p, n, v, s); }
+void fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) { fill(p, n, v, s); }
+
+// ---------------------------------------------------------------- scan
+// Three-phase exclusive scan: per-1024 block scan, a one-block scan of the
+// block totals (serial over 1024-wide chunks), then the block offsets added.
+// Integer arithmetic, so the result is exact and deterministic.
+constexpr int SCAN_B = 1024;
+
+__device__ __forceinline__ uint64_t block_incl_scan(uint64_t v, uint64_t* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_B; o <<= 1) {
+        const uint64_t a = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    return sh[t];
+}
+
+__global__ __launch_bounds__(SCAN_B) void k_scan_blocks(const uint32_t* __restrict__ in,
+                                                         uint64_t* __restrict__ tmp,
+                                                         uint64_t* __restrict__ bsum, size_t n) {
+    __shared__ uint64_t sh[SCAN_B];
+    const size_t i = (size_t)blockIdx.x * SCAN_B + threadIdx.x;
+    const uint64_t v = i < n ? in[i] : 0;
+    const uint64_t inc = block_incl_scan(v, sh);
+    if (i < n) tmp[i] = inc - v;
+    if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = inc;
+}
+
+__global__ __launch_bounds__(SCAN_B) void k_scan_sums(uint64_t* __restrict__ bsum, size_t nb,
+                                                       uint64_t* __restrict__ total) {
+    __shared__ uint64_t sh[SCAN_B];
+    uint64_t carry = 0;
+    for (size_t base = 0; base < nb; base += SCAN_B) {
+        const size_t i = base + threadIdx.x;
+        const uint64_t v = i < nb ? bsum[i] : 0;
+        const uint64_t inc = block_incl_scan(v, sh);
+        if (i < nb) bsum[i] = carry + inc - v;
+        carry += sh[SCAN_B - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_add(const uint64_t* __restrict__ tmp, const uint64_t* __restrict__ bsum,
+                           uint32_t* __restrict__ out, size_t n, const uint64_t* total) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)(tmp[i] + bsum[i / SCAN_B]);
+    if (i == n) out[n] = (uint32_t)*total;
+}
+
+uint64_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, int dev, hipStream_t s) {
+    const size_t nb = std::max<size_t>(1, cdiv(n, SCAN_B));
+    DBuf<uint64_t> tmp(dev, std::max<size_t>(n, 1)), bsum(dev, nb), total(dev, 1);
+    if (n) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_B), 0, s, in, tmp.p, bsum.p, n);
+    else fill(bsum.p, 1, (uint64_t)0, s);
+    FH_LAUNCH_CHECK("scan_blocks");
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, bsum.p, n ? nb : 1, total.p);
+    FH_LAUNCH_CHECK("scan_sums");
+    hipLaunchKernelGGL(k_scan_add, dim3(cdiv(n + 1, 256)), dim3(256), 0, s, tmp.p, bsum.p, out, n,
+                       total.p);
+    FH_LAUNCH_CHECK("scan_add");
+    uint64_t t = 0;
+    FH_HIP(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (t > 0xffffffffull) throw Error(FH_ERR_UNSUPPORTED, "index range exceeds 32 bits");
+    return t;
+}
+
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+int fh_device_count(int* count) {
+    return guard([&] {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+        *count = c;
+    });
+}
+
+int fh_get_device_info(int index, fh_device_info* out) {
+    return guard([&] {
+        FH_CHECK(out, "null output");
+        DeviceCtx& d = device(index);
+        std::memset(out, 0, sizeof *out);
+        out->index = index;
+        std::snprintf(out->name, sizeof out->name, "%s", d.prop.name);
+        std::snprintf(out->arch, sizeof out->arch, "%s", d.prop.gcnArchName);
+        size_t fr = 0, tot = 0;
+        FH_HIP(hipMemGetInfo(&fr, &tot));
+        out->total_memory_mib = tot >> 20;
+        out->free_memory_mib = fr >> 20;
+        out->compute_units = d.n_cu;
+        out->clock_mhz = d.prop.clockRate / 1000;
+    });
+}
+
+int fh_best_device(int* index) {
+    return guard([&] { *index = resolve_device(-1); });
+}
+
+int fh_synchronize(int dev) {
+    return guard([&] {
+        DeviceCtx& d = device(dev);
+        FH_HIP(hipStreamSynchronize(d.stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// fh_internal.hpp — device-resident objects behind the opaque C handles.
+#pragma once
+#include <memory>
+
+#include "fh_common.hpp"
+
+namespace fh {
+
+constexpr int SLICE = 64;  // rows per SELL slice == one wavefront
+
+// A symbolic sparsity pattern over "block rows" (a vertex or P2 dof), with the
+// CSR arrays (for export) and the SELL-64 slice layout (for the kernels).
+struct Pattern {
+    int dev = 0;
+    uint32_t nrows = 0;     // block rows
+    uint64_t nnzb = 0;      // stored blocks (CSR)
+    uint32_t nslices = 0;
+    uint64_t sell_entries = 0;  // padded entries (64 * sum of slice widths)
+    DBuf<uint32_t> row_ptr;     // [nrows+1] CSR
+    DBuf<uint32_t> cols;        // [nnzb] CSR, ascending per row
+    DBuf<uint32_t> slice_off;   // [nslices+1] entry offset of each slice
+    DBuf<uint32_t> sell_cols;   // [sell_entries]
+    DBuf<uint32_t> row_len;     // [nrows]
+    uint32_t max_width = 0;
+};
+
+// Dof -> cell adjacency of a cell-dof table (K dofs per cell), every dof's
+// cells in ascending order, plus for each (dof, cell) pair the position of
+// every cell dof inside that dof's pattern row ("slot map").
+struct Adjacency {
+    uint32_t ndofs = 0;
+    int K = 0;
+    DBuf<uint32_t> ptr;    // [ndofs+1]
+    DBuf<uint32_t> ent;    // cell * K + local
+    DBuf<uint8_t> slots;   // [entries][K] position in the pattern row
+};
+
+struct MeshP2 {
+    uint32_t nedges = 0;
+    DBuf<uint32_t> cell_dofs;  // [nc][10]
+    std::unique_ptr<Adjacency> adj;
+    std::shared_ptr<Pattern> pat;
+};
+
+}  // namespace fh
+
+struct fh_mesh {
+    int dev = 0;
+    int dim = 2;  // 2 (triangles) or 3 (tetrahedra)
+    int k = 3;    // vertices per cell
+    uint32_t nv = 0, nc = 0;
+    fh::DBuf<double> verts;   // [nv][dim]
+    fh::DBuf<uint32_t> cells; // [nc][k]
+    // lazily built
+    std::unique_ptr<fh::Adjacency> adj;
+    std::shared_ptr<fh::Pattern> pat;   // vertex graph pattern
+    std::unique_ptr<fh::MeshP2> p2;
+};
+
+struct fh_matrix {
+    int dev = 0;
+    int bs = 1;            // block size (1 scalar, 3 elasticity)
+    uint32_t n = 0;        // scalar rows
+    std::shared_ptr<fh::Pattern> pat;
+    fh::DBuf<double> vals;   // SELL layout, [entries][bs*bs] interleaved by lane
+    fh::DBuf<float> vals32;  // FP32 copy (lazily made)
+    fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
+    uint64_t nnz_scalar = 0; // canonical CSC nnz (export)
+    bool condensed = false;
+    fh::DBuf<uint32_t> fixed_count;  // duplicates in the prescribed list (diag value)
+};
+
+namespace fh {
+
+// quadrature rules of quadrature.rs:83-219 (points in reference coordinates)
+struct QuadTab {
+    double pts[5][3];
+    double w[5];
+    int n;
+};
+QuadTab quad_table(int dim, int degree);
+std::vector<uint64_t> mask_to_sorted(int dev, const uint8_t* mask, size_t n, hipStream_t s);
+
+// mesh / pattern (fh_mesh.hip, fh_pattern.hip)
+Adjacency& mesh_adjacency(fh_mesh* m);
+std::shared_ptr<Pattern> mesh_pattern(fh_mesh* m);
+MeshP2& mesh_p2(fh_mesh* m);
+std::unique_ptr<Adjacency> build_adjacency(int dev, const uint32_t* cell_dofs, uint32_t nc,
+                                           int K, uint32_t ndofs, hipStream_t s);
+std::shared_ptr<Pattern> build_pattern(int dev, const uint32_t* cell_dofs, int K,
+                                       Adjacency& adj, hipStream_t s);
+void build_slots(const uint32_t* cell_dofs, int K, Adjacency& adj, const Pattern& pat,
+                 hipStream_t s);
+void boundary_vertex_mask(fh_mesh* m, uint8_t* mask_dev, hipStream_t s,
+                          const uint32_t* cd10, uint8_t* mask2);
+
+// matrices (fh_assemble.hip)
+fh_matrix* assemble_p1(fh_mesh* m, int kind);  // 0 stiffness, 1 mass
+fh_matrix* assemble_elasticity(fh_mesh* m, double E, double nu, const double* Ecell_dev);
+void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ecell_dev,
+                              fh_matrix* a, hipStream_t s);
+void rhs_p1_dev(fh_mesh* m, const double* fq, double* b, hipStream_t s);
+void body_force_dev(fh_mesh* m, const double f[3], double* b, hipStream_t s);
+fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
+                           const uint64_t* dofs, const double* vals, double* b_out,
+                           hipStream_t s);
+fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_ptr,
+                           const uint32_t* cols, const double* vals);
+void ensure_vals32(fh_matrix* a);
+uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
+
+// SpMV (fh_cg.hip)
+void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStream_t s);
+
+// Solve (fh_cg.hip)
+struct SolveArgs {
+    const double* b;     // device
+    const double* x0;    // device or null
+    double* x;           // device out
+    double tol;
+    uint64_t max_iter;
+    int precision;
+    int order;           // FH_ORDER_*
+    const double* inv_diag_override = nullptr;  // device, scalar Jacobi
+    const double* block_dinv = nullptr;         // device, [nrows][9] block Jacobi
+};
+int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st);
+
+double spmv_algorithmic_bytes(const fh_matrix* a, int vbytes, int mbytes);
+
+}  // namespace fh

@@ -1,0 +1,735 @@
+"""fenics_hip — Python mirror of fenics-rs's assemble/solve API over the
+libfenics_hip C ABI (include/fenics_hip.h), running on MI355X (gfx950).
+
+Names, argument meaning and error behaviour follow the reference crates:
+
+* meshes: ``unit_square``, ``box_beam``, ``unit_cube``
+  (crates/fenics-mesh/src/generators.rs:24-117) built directly on the GPU;
+* assembly: ``assemble_stiffness_2d/3d``, ``assemble_rhs_2d/3d``,
+  ``apply_dirichlet_bc``, ``boundary_dofs_2d/3d`` (fenics-assembly/src/poisson.rs),
+  ``assemble_elasticity_stiffness_3d``, ``assemble_elasticity_body_force_3d``,
+  ``apply_vector_dirichlet_bc`` (fenics-assembly/src/elasticity.rs);
+* solvers: ``solve_cg`` (fenics-solver/src/cg.rs:40), ``solve_cg_gpu_or_cpu``,
+  ``solve_csr_cg_gpu``, ``solve_bsr3_pcg_gpu``, ``GpuBsr3Context``,
+  ``spmv_bsr3_multi_gpu``, ``discover_gpus``, ``best_gpu_index``
+  (fenics-solver/src/gpu.rs) and the drivers ``solve_poisson_2d/3d``,
+  ``solve_elasticity_3d``.
+
+Errors that the reference returns as ``Err(String)`` raise :class:`FenicsError`
+(non-convergence and CG breakdown included).  There is no CPU fallback: when
+the shared library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfenics_hip.so")
+
+FH_OK = 0
+FH_ERR_NO_DEVICE = 1
+FH_ERR_NOT_CONVERGED = 2
+FH_ERR_BREAKDOWN = 3
+FH_ERR_BAD_INPUT = 4
+FH_ERR_HIP = 5
+FH_ERR_UNSUPPORTED = 6
+
+FP64, MIXED, MIXED_IR = 0, 1, 2
+ORDER_FAST, ORDER_ORDERED = 0, 1
+
+
+class FenicsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("index", C.c_int), ("name", C.c_char * 128), ("arch", C.c_char * 64),
+                ("total_memory_mib", C.c_uint64), ("free_memory_mib", C.c_uint64),
+                ("compute_units", C.c_int), ("clock_mhz", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("replacements", C.c_uint64),
+                ("final_residual", C.c_double), ("rel_residual", C.c_double),
+                ("true_rel_residual", C.c_double), ("wall_ms", C.c_double),
+                ("kernel_ms", C.c_double), ("gflops", C.c_double),
+                ("effective_bw_gbps", C.c_double), ("spmv_bytes", C.c_double),
+                ("iter_bytes", C.c_double), ("spmv_avg_ms", C.c_double),
+                ("spmv_timed", C.c_uint64), ("device", C.c_int), ("precision", C.c_int),
+                ("status", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+vp = C.c_void_p
+u64 = C.c_uint64
+dbl = C.c_double
+PP = C.POINTER(C.c_void_p)
+
+
+def lib():
+    """Load libfenics_hip.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FenicsError(FH_ERR_UNSUPPORTED,
+                          f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "fh_last_error": (C.c_char_p, []),
+        "fh_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "fh_get_device_info": (C.c_int, [C.c_int, C.POINTER(DeviceInfo)]),
+        "fh_best_device": (C.c_int, [C.POINTER(C.c_int)]),
+        "fh_synchronize": (C.c_int, [C.c_int]),
+        "fh_mesh_create": (C.c_int, [C.c_int, u64, vp, u64, vp, C.c_int, PP]),
+        "fh_mesh_unit_square": (C.c_int, [u64, u64, C.c_int, PP]),
+        "fh_mesh_box_beam": (C.c_int, [dbl, dbl, dbl, u64, u64, u64, C.c_int, PP]),
+        "fh_mesh_sizes": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(u64)]),
+        "fh_mesh_download": (C.c_int, [vp, vp, vp]),
+        "fh_mesh_boundary_dofs": (C.c_int, [vp, vp, C.POINTER(u64)]),
+        "fh_quad_points": (C.c_int, [vp, C.c_int, vp]),
+        "fh_mesh_destroy": (C.c_int, [vp]),
+        "fh_assemble_stiffness": (C.c_int, [vp, PP]),
+        "fh_assemble_mass": (C.c_int, [vp, PP]),
+        "fh_assemble_elasticity": (C.c_int, [vp, dbl, dbl, vp, PP]),
+        "fh_assemble_elasticity_dev": (C.c_int, [vp, dbl, dbl, vp, PP]),
+        "fh_assemble_p2_stiffness": (C.c_int, [vp, C.c_int, PP]),
+        "fh_p2_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "fh_p2_cell_dofs": (C.c_int, [vp, vp]),
+        "fh_p2_boundary_dofs": (C.c_int, [vp, vp, C.POINTER(u64)]),
+        "fh_assemble_rhs": (C.c_int, [vp, vp, vp]),
+        "fh_assemble_p2_rhs_cellconst": (C.c_int, [vp, vp, vp]),
+        "fh_assemble_body_force": (C.c_int, [vp, vp, vp]),
+        "fh_assemble_body_force_dev": (C.c_int, [vp, vp, vp]),
+        "fh_apply_dirichlet": (C.c_int, [vp, vp, u64, vp, vp, PP, vp]),
+        "fh_apply_dirichlet_dev": (C.c_int, [vp, vp, u64, vp, vp, PP, vp]),
+        "fh_matrix_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_int)]),
+        "fh_matrix_export_csc": (C.c_int, [vp, vp, vp, vp]),
+        "fh_matrix_spmv": (C.c_int, [vp, vp, vp, C.c_int]),
+        "fh_matrix_spmv_dev": (C.c_int, [vp, vp, vp, C.c_int]),
+        "fh_matrix_destroy": (C.c_int, [vp]),
+        "fh_solve": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
+        "fh_solve_dev": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
+        "fh_solve_csc_cg": (C.c_int, [u64, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
+                                      C.POINTER(SolveStats)]),
+        "fh_solve_csr_cg": (C.c_int, [u64, vp, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
+                                      C.POINTER(SolveStats)]),
+        "fh_solve_bsr3_pcg": (C.c_int, [u64, vp, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
+                                        C.POINTER(SolveStats)]),
+        "fh_bsr3_ctx_create": (C.c_int, [u64, vp, vp, vp, vp, C.c_int, PP]),
+        "fh_bsr3_ctx_spmv": (C.c_int, [vp, vp, vp]),
+        "fh_bsr3_ctx_jacobi_smooth": (C.c_int, [vp, vp, vp, u64, dbl, C.c_int]),
+        "fh_bsr3_ctx_device_name": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
+        "fh_bsr3_ctx_nnz_blocks": (C.c_int, [vp, C.POINTER(u64)]),
+        "fh_bsr3_ctx_destroy": (C.c_int, [vp]),
+        "fh_spmv_bsr3_multi": (C.c_int, [u64, vp, vp, vp, vp, u64, vp, C.POINTER(C.c_int)]),
+        "fh_solve_poisson": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
+        "fh_solve_elasticity_3d": (C.c_int, [vp, dbl, dbl, vp, vp, u64, vp, vp, dbl, u64,
+                                             C.c_int, vp, vp, C.POINTER(SolveStats)]),
+        "fh_cell_bilinear_k0_dev": (C.c_int, [vp, dbl, vp, vp, vp]),
+        "fh_cell_bilinear_k0": (C.c_int, [vp, dbl, vp, vp, vp]),
+        "fh_element_stresses": (C.c_int, [vp, vp, dbl, dbl, vp]),
+        "fh_vertex_von_mises": (C.c_int, [vp, vp, vp]),
+        "fh_batched_poisson_block_cg": (C.c_int, [u64, vp, u64, vp, dbl, u64, C.c_int, C.c_int,
+                                                  vp, vp, C.POINTER(SolveStats)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().fh_last_error().decode(errors="replace")
+
+
+def _check(code: int, allow=()):
+    if code != FH_OK and code not in allow:
+        raise FenicsError(code, last_error())
+    return code
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ------------------------------------------------------------------ devices
+@dataclasses.dataclass
+class GpuDeviceInfo:
+    """gpu.rs:12-21 GpuDeviceInfo."""
+    index: int
+    name: str
+    arch: str
+    total_memory_mib: int
+    free_memory_mib: int
+    multiprocessor_count: int
+
+
+def discover_gpus():
+    """gpu.rs:178-234.  Empty list when no HIP device is visible."""
+    n = C.c_int(0)
+    lib().fh_device_count(C.byref(n))
+    out = []
+    for i in range(n.value):
+        info = DeviceInfo()
+        if lib().fh_get_device_info(i, C.byref(info)) == FH_OK:
+            out.append(GpuDeviceInfo(i, info.name.decode(), info.arch.decode(),
+                                     info.total_memory_mib, info.free_memory_mib,
+                                     info.compute_units))
+    return out
+
+
+def best_gpu_index() -> Optional[int]:
+    """gpu.rs:237-242: the device with the most free memory."""
+    i = C.c_int(-1)
+    return i.value if lib().fh_best_device(C.byref(i)) == FH_OK else None
+
+
+# ------------------------------------------------------------------ meshes
+class Mesh:
+    """Device-resident SimplicialMesh (mesh.rs:10-20)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+        d, nv, nc = C.c_int(), u64(), u64()
+        _check(lib().fh_mesh_sizes(self.h, C.byref(d), C.byref(nv), C.byref(nc)))
+        self.dim, self.nv, self.nc = d.value, int(nv.value), int(nc.value)
+        self._cache = {}
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fh_mesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def from_arrays(vertices, cells, device=-1) -> "Mesh":
+        v, c = _f64(vertices), _u64(cells)
+        h = C.c_void_p()
+        _check(lib().fh_mesh_create(v.shape[1], v.shape[0], _p(v), c.shape[0], _p(c), device,
+                                    C.byref(h)))
+        return Mesh(h.value)
+
+    def num_vertices(self):
+        return self.nv
+
+    def num_cells(self):
+        return self.nc
+
+    @property
+    def vertices(self) -> np.ndarray:
+        if "v" not in self._cache:
+            self._download()
+        return self._cache["v"]
+
+    @property
+    def cells(self) -> np.ndarray:
+        if "c" not in self._cache:
+            self._download()
+        return self._cache["c"]
+
+    def _download(self):
+        v = np.empty((self.nv, self.dim))
+        c = np.empty((self.nc, self.dim + 1), np.uint64)
+        _check(lib().fh_mesh_download(self.h, _p(v), _p(c)))
+        self._cache["v"], self._cache["c"] = v, c
+
+    def boundary_dofs(self) -> np.ndarray:
+        """poisson.rs:266-287 boundary_dofs_2d/3d (sorted)."""
+        n = u64(0)
+        _check(lib().fh_mesh_boundary_dofs(self.h, None, C.byref(n)))
+        out = np.empty(n.value, np.uint64)
+        _check(lib().fh_mesh_boundary_dofs(self.h, _p(out), C.byref(n)))
+        return out
+
+    def quad_points(self, degree: int = 2) -> np.ndarray:
+        nq = {(2, 1): 1, (2, 2): 3, (2, 3): 4, (3, 1): 1, (3, 2): 4, (3, 3): 5}[(self.dim, min(degree, 3))]
+        out = np.empty((self.nc, nq, self.dim))
+        _check(lib().fh_quad_points(self.h, degree, _p(out)))
+        return out
+
+    # P2 tetrahedral dofs (vertices, then edges in lexicographic order)
+    def p2_info(self):
+        nd, ne = u64(), u64()
+        _check(lib().fh_p2_info(self.h, C.byref(nd), C.byref(ne)))
+        return int(nd.value), int(ne.value)
+
+    def p2_cell_dofs(self):
+        out = np.empty((self.nc, 10), np.uint64)
+        _check(lib().fh_p2_cell_dofs(self.h, _p(out)))
+        return out
+
+    def p2_boundary_dofs(self):
+        n = u64(0)
+        _check(lib().fh_p2_boundary_dofs(self.h, None, C.byref(n)))
+        out = np.empty(n.value, np.uint64)
+        _check(lib().fh_p2_boundary_dofs(self.h, _p(out), C.byref(n)))
+        return out
+
+
+def unit_square(nx: int, ny: int, device: int = -1) -> Mesh:
+    h = C.c_void_p()
+    _check(lib().fh_mesh_unit_square(nx, ny, device, C.byref(h)))
+    return Mesh(h.value)
+
+
+def box_beam(length, width, height, nx, ny, nz, device: int = -1) -> Mesh:
+    h = C.c_void_p()
+    _check(lib().fh_mesh_box_beam(float(length), float(width), float(height), nx, ny, nz, device,
+                                  C.byref(h)))
+    return Mesh(h.value)
+
+
+def unit_cube(nx, ny, nz, device: int = -1) -> Mesh:
+    return box_beam(1.0, 1.0, 1.0, nx, ny, nz, device)
+
+
+def boundary_dofs_2d(mesh: Mesh) -> np.ndarray:
+    return mesh.boundary_dofs()
+
+
+boundary_dofs_3d = boundary_dofs_2d
+
+
+# ------------------------------------------------------------------ matrices
+class SparseMatrix:
+    """Device-resident assembled operator; exports faer's SparseColMat raw parts."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fh_matrix_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _info(self):
+        n, nnz, bs = u64(), u64(), C.c_int()
+        _check(lib().fh_matrix_info(self.h, C.byref(n), C.byref(nnz), C.byref(bs)))
+        return int(n.value), int(nnz.value), bs.value
+
+    def nrows(self):
+        return self._info()[0]
+
+    ncols = nrows
+
+    @property
+    def block(self):
+        return self._info()[2]
+
+    def nnz(self):
+        return self._info()[1]
+
+    def export_csc(self):
+        """(col_ptr u64[n+1], row_idx u64[nnz], values f64[nnz]) — faer's canonical CSC."""
+        n = self.nrows()
+        cp = np.empty(n + 1, np.uint64)
+        _check(lib().fh_matrix_export_csc(self.h, _p(cp), None, None))
+        nnz = int(cp[-1])
+        ri = np.empty(nnz, np.uint64)
+        va = np.empty(nnz)
+        _check(lib().fh_matrix_export_csc(self.h, _p(cp), _p(ri), _p(va)))
+        return cp, ri, va
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        cp, ri, va = self.export_csc()
+        n = len(cp) - 1
+        return sp.csc_matrix((va, ri.astype(np.int64), cp.astype(np.int64)), shape=(n, n))
+
+    def spmv(self, x, order: int = ORDER_FAST) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty_like(x)
+        _check(lib().fh_matrix_spmv(self.h, _p(x), _p(y), order))
+        return y
+
+    def solve(self, b, x0=None, tol=1e-10, max_iter=1000, precision=FP64, allow_fail=False):
+        """Jacobi-PCG (cg.rs:40-149).  Returns (x, SolveStats)."""
+        b = _f64(b)
+        x = np.empty_like(b)
+        x0a = None if x0 is None else _f64(x0)
+        st = SolveStats()
+        code = lib().fh_solve(self.h, _p(b), _p(x0a), tol, max_iter, precision, _p(x),
+                              C.byref(st))
+        if not allow_fail:
+            _check(code)
+        elif code not in (FH_OK, FH_ERR_NOT_CONVERGED, FH_ERR_BREAKDOWN):
+            _check(code)
+        return x, st
+
+
+def _new_matrix(fn, *args) -> SparseMatrix:
+    h = C.c_void_p()
+    _check(fn(*args, C.byref(h)))
+    return SparseMatrix(h.value)
+
+
+def assemble_stiffness(mesh: Mesh) -> SparseMatrix:
+    """poisson.rs:23-69 / 124-165."""
+    return _new_matrix(lib().fh_assemble_stiffness, mesh.h)
+
+
+assemble_stiffness_2d = assemble_stiffness
+assemble_stiffness_3d = assemble_stiffness
+
+
+def assemble_mass(mesh: Mesh) -> SparseMatrix:
+    """mass.rs:10-82."""
+    return _new_matrix(lib().fh_assemble_mass, mesh.h)
+
+
+def _sample(mesh: Mesh, f, degree=2):
+    if callable(f):
+        xq = mesh.quad_points(degree)
+        vals = np.asarray(f(xq.reshape(-1, mesh.dim).T), dtype=np.float64)
+        return _f64(np.broadcast_to(vals, (xq.shape[0] * xq.shape[1],)).reshape(xq.shape[:2]))
+    return _f64(f)
+
+
+def assemble_rhs(mesh: Mesh, f) -> np.ndarray:
+    """poisson.rs:72-121 / 168-218.  `f` maps coordinates (dim x N array) to
+    values, or is the array of values at mesh.quad_points(2)."""
+    fq = _sample(mesh, f)
+    b = np.empty(mesh.nv)
+    _check(lib().fh_assemble_rhs(mesh.h, _p(fq), _p(b)))
+    return b
+
+
+assemble_rhs_2d = assemble_rhs
+assemble_rhs_3d = assemble_rhs
+
+
+@dataclasses.dataclass
+class DirichletBC:
+    """poisson.rs:13-20."""
+    dofs: Sequence[int]
+    values: Sequence[float]
+
+
+@dataclasses.dataclass
+class VectorDirichletBC:
+    """elasticity.rs:63-69: list of (dof, value)."""
+    prescribed_dofs: Sequence[tuple]
+
+
+def _apply(mat: SparseMatrix, rhs, dofs, vals):
+    rhs = _f64(rhs)
+    dofs, vals = _u64(dofs), _f64(vals)
+    out = np.empty_like(rhs)
+    h = C.c_void_p()
+    _check(lib().fh_apply_dirichlet(mat.h, _p(rhs), len(dofs), _p(dofs), _p(vals), C.byref(h),
+                                    _p(out)))
+    return SparseMatrix(h.value), out
+
+
+def apply_dirichlet_bc(mat: SparseMatrix, rhs, bc: DirichletBC):
+    """poisson.rs:222-263."""
+    return _apply(mat, rhs, bc.dofs, bc.values)
+
+
+def apply_vector_dirichlet_bc(mat: SparseMatrix, rhs, bc: VectorDirichletBC):
+    """elasticity.rs:188-227."""
+    pd = list(bc.prescribed_dofs)
+    dofs = [d for d, _ in pd]
+    vals = [v for _, v in pd]
+    return _apply(mat, rhs, dofs, vals)
+
+
+@dataclasses.dataclass
+class ElasticMaterial:
+    """elasticity.rs:14-48."""
+    youngs_modulus: float
+    poissons_ratio: float
+
+    def __post_init__(self):
+        if not (-1.0 < self.poissons_ratio < 0.5):
+            raise ValueError("Poisson's ratio must be in (-1.0, 0.5) for physical stability")
+
+    def lame_lambda(self):
+        e, nu = self.youngs_modulus, self.poissons_ratio
+        return (e * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+    def lame_mu(self):
+        e, nu = self.youngs_modulus, self.poissons_ratio
+        return e / (2.0 * (1.0 + nu))
+
+
+def assemble_elasticity_stiffness_3d(mesh: Mesh, material: ElasticMaterial,
+                                     E_cell=None) -> SparseMatrix:
+    """elasticity.rs:73-149 (E_cell: optional per-cell Young's moduli)."""
+    ec = None if E_cell is None else _f64(E_cell)
+    return _new_matrix(lib().fh_assemble_elasticity, mesh.h, material.youngs_modulus,
+                       material.poissons_ratio, _p(ec))
+
+
+def assemble_elasticity_body_force_3d(mesh: Mesh, f_body) -> np.ndarray:
+    """elasticity.rs:153-185."""
+    f = _f64(f_body)
+    b = np.empty(3 * mesh.nv)
+    _check(lib().fh_assemble_body_force(mesh.h, _p(f), _p(b)))
+    return b
+
+
+def assemble_p2_stiffness(mesh: Mesh, method: int = 0) -> SparseMatrix:
+    """P2 tetrahedral Poisson stiffness (method 0: FP64 VALU, 1: FP32 MFMA)."""
+    return _new_matrix(lib().fh_assemble_p2_stiffness, mesh.h, method)
+
+
+def assemble_p2_rhs_cellconst(mesh: Mesh, f_cell) -> np.ndarray:
+    f = _f64(f_cell)
+    nd, _ = mesh.p2_info()
+    b = np.empty(nd)
+    _check(lib().fh_assemble_p2_rhs_cellconst(mesh.h, _p(f), _p(b)))
+    return b
+
+
+# ------------------------------------------------------------------ solvers
+def solve_cg(mat: SparseMatrix, b, x0=None, tol=1e-10, max_iter=1000, precision=FP64):
+    """cg.rs:40-149: returns (x, iterations, final absolute residual);
+    raises FenicsError on breakdown / non-convergence like the reference's Err."""
+    x, st = mat.solve(b, x0, tol, max_iter, precision)
+    return x, int(st.iterations), st.final_residual
+
+
+@dataclasses.dataclass
+class GpuSolveResult:
+    """gpu.rs:24-34."""
+    solution: np.ndarray
+    iterations: int
+    final_residual: float
+    wall_ms: float
+    kernel_ms: float
+    gflops: float
+    effective_bw_gbps: float
+    devices_used: list
+    stats: SolveStats = None
+
+
+def _result(x, st: SolveStats):
+    infos = discover_gpus()
+    name = infos[st.device].name if st.device < len(infos) else f"device {st.device}"
+    return GpuSolveResult(x, int(st.iterations), st.rel_residual, st.wall_ms, st.kernel_ms,
+                          st.gflops, st.effective_bw_gbps, [name], st)
+
+
+def solve_cg_gpu_or_cpu(col_ptr, row_idx, values, b, x0=None, tol=1e-10, max_iter=1000,
+                        device=-1) -> GpuSolveResult:
+    """gpu.rs:1542-1602 on faer's raw CSC parts (no CPU fallback here)."""
+    cp, ri, va, b = _u64(col_ptr), _u64(row_idx), _f64(values), _f64(b)
+    x = np.empty_like(b)
+    x0a = None if x0 is None else _f64(x0)
+    st = SolveStats()
+    _check(lib().fh_solve_csc_cg(len(cp) - 1, _p(cp), _p(ri), _p(va), _p(b), _p(x0a), tol,
+                                 max_iter, device, _p(x), C.byref(st)))
+    return _result(x, st)
+
+
+def solve_csr_cg_gpu(row_ptr, col_idx, vals, inv_diag, b, x0=None, rtol=1e-10, max_iter=1000,
+                     device_index=None) -> GpuSolveResult:
+    """gpu.rs:1145-1442."""
+    rp, ci, va, inv, b = _u32(row_ptr), _u32(col_idx), _f64(vals), _f64(inv_diag), _f64(b)
+    x = np.empty_like(b)
+    x0a = None if x0 is None else _f64(x0)
+    st = SolveStats()
+    dev = -1 if device_index is None else device_index
+    _check(lib().fh_solve_csr_cg(len(rp) - 1, _p(rp), _p(ci), _p(va), _p(inv), _p(b), _p(x0a),
+                                 rtol, max_iter, dev, _p(x), C.byref(st)))
+    return _result(x, st)
+
+
+def solve_bsr3_pcg_gpu(n_nodes, row_ptr, cols, vals_flat, dinv_flat, b, x0=None, rtol=1e-10,
+                       max_iter=1000, device_index=None) -> GpuSolveResult:
+    """gpu.rs:775-1141 (3x3 block CSR, 3x3 block-Jacobi)."""
+    rp, cc, va, di, b = _u32(row_ptr), _u32(cols), _f64(vals_flat), _f64(dinv_flat), _f64(b)
+    x = np.empty_like(b)
+    x0a = None if x0 is None else _f64(x0)
+    st = SolveStats()
+    dev = -1 if device_index is None else device_index
+    _check(lib().fh_solve_bsr3_pcg(n_nodes, _p(rp), _p(cc), _p(va), _p(di), _p(b), _p(x0a),
+                                   rtol, max_iter, dev, _p(x), C.byref(st)))
+    return _result(x, st)
+
+
+class GpuBsr3Context:
+    """gpu.rs:432-658: persistent device-resident 3x3 BSR operator."""
+
+    def __init__(self, n_nodes, row_ptr, cols, vals_flat, dinv_flat, device_index=None):
+        self._keep = (_u32(row_ptr), _u32(cols), _f64(vals_flat), _f64(dinv_flat))
+        rp, cc, va, di = self._keep
+        self.n = 3 * n_nodes
+        h = C.c_void_p()
+        dev = -1 if device_index is None else device_index
+        _check(lib().fh_bsr3_ctx_create(n_nodes, _p(rp), _p(cc), _p(va), _p(di), dev, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            lib().fh_bsr3_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+    def spmv(self, x) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty(self.n)
+        _check(lib().fh_bsr3_ctx_spmv(self.h, _p(x), _p(y)))
+        return y
+
+    def jacobi_smooth(self, r, z, sweeps, omega, zero_initial_z) -> np.ndarray:
+        r = _f64(r)
+        z = np.array(z, dtype=np.float64, copy=True)
+        _check(lib().fh_bsr3_ctx_jacobi_smooth(self.h, _p(r), _p(z), sweeps, omega,
+                                               int(bool(zero_initial_z))))
+        return z
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(128)
+        _check(lib().fh_bsr3_ctx_device_name(self.h, buf, 128))
+        return buf.value.decode()
+
+    def nnz_blocks(self) -> int:
+        v = u64()
+        _check(lib().fh_bsr3_ctx_nnz_blocks(self.h, C.byref(v)))
+        return int(v.value)
+
+
+def spmv_bsr3_multi_gpu(n_nodes, row_ptr, cols, vals_flat, x, max_gpus):
+    """gpu.rs:662-771: returns (y, device names used)."""
+    rp, cc, va, x = _u32(row_ptr), _u32(cols), _f64(vals_flat), _f64(x)
+    y = np.empty(3 * n_nodes)
+    used = C.c_int()
+    _check(lib().fh_spmv_bsr3_multi(n_nodes, _p(rp), _p(cc), _p(va), _p(x), max_gpus, _p(y),
+                                    C.byref(used)))
+    names = [g.name for g in discover_gpus()][: used.value]
+    return y, names
+
+
+# ------------------------------------------------------------------ drivers
+@dataclasses.dataclass
+class PoissonSolution:
+    """poisson_solver.rs:16-23."""
+    u: np.ndarray
+    iterations: int
+    residual: float
+    stats: SolveStats = None
+
+
+def _solve_poisson(mesh: Mesh, source, boundary_value, tol, max_iter, precision):
+    fq = _sample(mesh, source)
+    bnd = mesh.boundary_dofs()
+    verts = mesh.vertices
+    if boundary_value is None:
+        g = np.zeros(len(bnd))
+    else:
+        g = _f64([boundary_value(verts[int(i)]) for i in bnd])
+    u = np.empty(mesh.nv)
+    st = SolveStats()
+    _check(lib().fh_solve_poisson(mesh.h, _p(fq), _p(g), tol, max_iter, precision, _p(u),
+                                  C.byref(st)))
+    return PoissonSolution(u, int(st.iterations), st.final_residual, st)
+
+
+def solve_poisson_2d(mesh, source, boundary_value, tol, max_iter, precision=FP64):
+    """poisson_solver.rs:26-59: source(x: dim x N array) -> values,
+    boundary_value(vertex) -> value."""
+    return _solve_poisson(mesh, source, boundary_value, tol, max_iter, precision)
+
+
+solve_poisson_3d = solve_poisson_2d
+
+
+@dataclasses.dataclass
+class ElasticitySolution:
+    """elasticity_solver.rs:12-28 (stresses filled by compute_stresses=True)."""
+    displacements: np.ndarray
+    iterations: int
+    residual: float
+    deformed_vertices: np.ndarray = None
+    element_stresses: np.ndarray = None
+    vertex_von_mises: np.ndarray = None
+    max_von_mises: float = 0.0
+    stats: SolveStats = None
+
+
+def solve_elasticity_3d(mesh: Mesh, material: ElasticMaterial, body_force,
+                        bc: VectorDirichletBC, tol, max_iter, precision=FP64, E_cell=None,
+                        compute_stresses=True) -> ElasticitySolution:
+    """elasticity_solver.rs:31-72."""
+    pd = list(bc.prescribed_dofs)
+    dofs = _u64([d for d, _ in pd])
+    vals = _f64([v for _, v in pd])
+    f = _f64(body_force)
+    ec = None if E_cell is None else _f64(E_cell)
+    u = np.empty(3 * mesh.nv)
+    st = SolveStats()
+    _check(lib().fh_solve_elasticity_3d(mesh.h, material.youngs_modulus, material.poissons_ratio,
+                                        _p(ec), _p(f), len(dofs), _p(dofs), _p(vals), tol,
+                                        max_iter, precision, _p(u), None, C.byref(st)))
+    sol = ElasticitySolution(u, int(st.iterations), st.final_residual, stats=st)
+    sol.deformed_vertices = mesh.vertices + u.reshape(-1, 3)
+    if compute_stresses and hasattr(lib(), "fh_element_stresses"):
+        sol.element_stresses = element_stresses(mesh, u, material)
+        sol.vertex_von_mises = vertex_von_mises(mesh, sol.element_stresses[:, 18])
+        sol.max_von_mises = float(max(0.0, sol.vertex_von_mises.max(initial=0.0)))
+    return sol
+
+
+def element_stresses(mesh: Mesh, u, material: ElasticMaterial) -> np.ndarray:
+    """elasticity.rs:230-320: per cell stress[9], strain[9], von_mises, trace."""
+    u = _f64(u)
+    out = np.empty((mesh.nc, 20))
+    _check(lib().fh_element_stresses(mesh.h, _p(u), material.youngs_modulus,
+                                     material.poissons_ratio, _p(out)))
+    return out
+
+
+def vertex_von_mises(mesh: Mesh, vm_cell) -> np.ndarray:
+    """elasticity.rs:323-360."""
+    vm = _f64(vm_cell)
+    out = np.empty(mesh.nv)
+    _check(lib().fh_vertex_von_mises(mesh.h, _p(vm), _p(out)))
+    return out
+
+
+def cell_bilinear_k0(mesh: Mesh, nu, w, u) -> np.ndarray:
+    """out[c] = -w_c^T K0_c u_c (E = 1 element stiffness); dJ/dE_c for w = u."""
+    w, u = _f64(w), _f64(u)
+    out = np.empty(mesh.nc)
+    _check(lib().fh_cell_bilinear_k0(mesh.h, float(nu), _p(w), _p(u), _p(out)))
+    return out
+
+
+def clamp_dofs_x0(vertices, tol=1e-6):
+    """elasticity_tests.rs:94-101: all three dofs of the vertices with x < 1e-6."""
+    idx = np.nonzero(np.asarray(vertices)[:, 0] < tol)[0]
+    return VectorDirichletBC([(3 * int(i) + a, 0.0) for i in idx for a in range(3)])

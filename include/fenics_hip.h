@@ -1,0 +1,234 @@
+/*
+ * fenics_hip.h — C ABI of libfenics_hip, the MI355X (gfx950) implementation
+ * of the fenics-rs assembly + conjugate-gradient hot path.
+ *
+ * Every entry point is plain C: pointers and sizes, no C++ or torch types.
+ * Conventions (mirroring fenics-rs, crates/fenics-solver/src/gpu.rs):
+ *   - every call returns an int status (FH_OK == 0) and never aborts across
+ *     the FFI; fh_last_error() returns a thread-local message for the last
+ *     failure (the Rust side maps it to Err(String), as gpu.rs:170-176 does
+ *     for CUDA errors);
+ *   - host arrays are borrowed for the duration of the call; outputs go to
+ *     caller-allocated buffers sized by a query-then-fill pattern;
+ *   - objects (fh_mesh, fh_matrix, fh_bsr3_ctx) own their device memory and
+ *     are used by one thread at a time (Send, not Sync — gpu.rs:450);
+ *   - a CPU fallback stays a caller decision (gpu.rs:1584-1601): this library
+ *     never computes on the host.
+ * Entry points whose names end in _dev take device pointers that must live
+ * on the object's device.  Indices at the CSC boundary are uint64_t
+ * (faer's usize); device-side indices are uint32_t.
+ */
+#ifndef FENICS_HIP_H
+#define FENICS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define FH_OK 0
+#define FH_ERR_NO_DEVICE 1
+#define FH_ERR_NOT_CONVERGED 2 /* cg.rs:139-148 "CG failed to converge"        */
+#define FH_ERR_BREAKDOWN 3     /* cg.rs:102-107 |p^T A p| < 1e-30                */
+#define FH_ERR_BAD_INPUT 4     /* shapes, indices out of range, bad material     */
+#define FH_ERR_HIP 5           /* HIP runtime error, message in fh_last_error()  */
+#define FH_ERR_UNSUPPORTED 6
+
+/* arithmetic of a solve */
+#define FH_PREC_FP64 0     /* FP64 matrix, vectors and reductions                     */
+#define FH_PREC_MIXED 1    /* FP32 CG with FP64 residual replacement (x, r, dots FP64) */
+#define FH_PREC_MIXED_IR 2 /* restarted FP32 CG inside FP64 iterative refinement       */
+
+/* SpMV accumulation order */
+#define FH_ORDER_FAST 0    /* fused multiply-add, any order within a row           */
+#define FH_ORDER_ORDERED 1 /* cg.rs:13-21 order: column-ascending, no FMA (bitwise) */
+
+/* replaces GpuDeviceInfo (gpu.rs:12-21) */
+typedef struct fh_device_info {
+    int index;
+    char name[128];
+    char arch[64]; /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    uint64_t total_memory_mib;
+    uint64_t free_memory_mib;
+    int compute_units; /* multiprocessor_count */
+    int clock_mhz;
+} fh_device_info;
+
+/* replaces GpuSolveResult (gpu.rs:24-34) minus the solution, which goes to x_out */
+typedef struct fh_solve_stats {
+    uint64_t iterations;          /* CG iterations (inner FP32 ones for the mixed modes) */
+    uint64_t replacements;        /* FP64 residual replacements / refinement steps       */
+    double final_residual;        /* cg.rs semantics: absolute recursive ||r||           */
+    double rel_residual;          /* recursive ||r|| / ||b|| (gpu.rs final_residual)     */
+    double true_rel_residual;     /* ||b - A x|| / ||b|| recomputed in FP64 on the GPU    */
+    double wall_ms;
+    double kernel_ms;
+    double gflops;
+    double effective_bw_gbps;     /* algorithmic bytes per iteration / time per iteration */
+    double spmv_bytes;            /* algorithmic bytes of one SpMV of this matrix        */
+    double iter_bytes;            /* algorithmic bytes of one CG iteration               */
+    double spmv_avg_ms;           /* mean duration of the CG SpMV kernel (HIP events), 0 if
+                                     not timed (set FH_TIME_SPMV=1 to time it)           */
+    uint64_t spmv_timed;          /* SpMV launches that average covers                   */
+    int device;
+    int precision;
+    int status;
+} fh_solve_stats;
+
+typedef struct fh_mesh fh_mesh;         /* device-resident simplicial mesh           */
+typedef struct fh_matrix fh_matrix;     /* device-resident assembled sparse operator */
+typedef struct fh_bsr3_ctx fh_bsr3_ctx; /* replaces GpuBsr3Context (gpu.rs:432-658)   */
+
+/* ------------------------------------------------------- devices, errors */
+const char* fh_last_error(void);
+int fh_device_count(int* count);
+int fh_get_device_info(int index, fh_device_info* out);      /* gpu.rs:178-234 discover_gpus */
+int fh_best_device(int* index);                           /* gpu.rs:237-242 best_gpu_index */
+int fh_synchronize(int device);
+
+/* ------------------------------------------------------------------ mesh
+ * replaces SimplicialMesh<DIM,NODES> (crates/fenics-mesh/src/mesh.rs:10-20);
+ * vertices AoS [nv][dim] f64, cells [nc][dim+1] u64 as in the reference.   */
+int fh_mesh_create(int dim, uint64_t nv, const double* vertices, uint64_t nc,
+                   const uint64_t* cells, int device, fh_mesh** out);
+/* generators.rs:24-55 unit_square, built on the device */
+int fh_mesh_unit_square(uint64_t nx, uint64_t ny, int device, fh_mesh** out);
+/* generators.rs:58-111 box_beam (unit_cube = box_beam(1,1,1,..)), on the device */
+int fh_mesh_box_beam(double L, double W, double H, uint64_t nx, uint64_t ny, uint64_t nz,
+                     int device, fh_mesh** out);
+int fh_mesh_sizes(const fh_mesh* m, int* dim, uint64_t* nv, uint64_t* nc);
+int fh_mesh_download(const fh_mesh* m, double* vertices, uint64_t* cells);
+/* poisson.rs:266-287 boundary_dofs_2d/3d: sorted vertex ids on boundary
+ * facets (mesh.rs:79-113,145-210).  out may be NULL to query the count.   */
+int fh_mesh_boundary_dofs(fh_mesh* m, uint64_t* out, uint64_t* count);
+/* physical points of the degree-`degree` rule, [nc][nq][dim]
+ * (map_to_physical, jacobian.rs:44-48,88-92); nq = 3/4 (2D/3D, degree 2)   */
+int fh_quad_points(fh_mesh* m, int degree, double* xq_out);
+int fh_mesh_destroy(fh_mesh* m);
+
+/* -------------------------------------------------------------- assembly
+ * Matrices keep faer's canonical CSC pattern of the connectivity (sorted
+ * rows per column, duplicates summed in ascending cell order, structural
+ * zeros kept); stored on the device in 64-row sliced ELL (SELL-64) with
+ * 1x1 or 3x3 blocks, FP64 values plus an FP32 copy for mixed solves.       */
+/* poisson.rs:23-69 / 124-165 assemble_stiffness_2d/3d (by mesh dim) */
+int fh_assemble_stiffness(fh_mesh* m, fh_matrix** out);
+/* mass.rs:10-44 / 47-82 assemble_mass_2d/3d */
+int fh_assemble_mass(fh_mesh* m, fh_matrix** out);
+/* elasticity.rs:73-149 assemble_elasticity_stiffness_3d; E_cell may be NULL
+ * (uniform E) or a host array [nc] of per-cell Young's moduli              */
+int fh_assemble_elasticity(fh_mesh* m, double E, double nu, const double* E_cell,
+                           fh_matrix** out);
+int fh_assemble_elasticity_dev(fh_mesh* m, double E, double nu, const double* E_cell_dev,
+                               fh_matrix** out);
+/* P2 tetrahedral Poisson stiffness (new): dofs = vertices, then edges in
+ * lexicographic (min,max) order; method 0 = VALU FP64 (exact order of the
+ * oracle), 1 = MFMA FP32 element matrices (FP32 accumulate)               */
+int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out);
+int fh_p2_info(fh_mesh* m, uint64_t* ndofs, uint64_t* nedges);
+int fh_p2_cell_dofs(fh_mesh* m, uint64_t* out /* [nc][10] */);
+int fh_p2_boundary_dofs(fh_mesh* m, uint64_t* out, uint64_t* count);
+/* poisson.rs:72-121 / 168-218 assemble_rhs_2d/3d with the source sampled at
+ * fh_quad_points(m, 2, ...): fq [nc][nq]                                   */
+int fh_assemble_rhs(fh_mesh* m, const double* fq, double* b_out);
+/* P2 load with a per-cell constant source f_cell [nc] (same rule)         */
+int fh_assemble_p2_rhs_cellconst(fh_mesh* m, const double* f_cell, double* b_out);
+/* elasticity.rs:153-185 assemble_elasticity_body_force_3d */
+int fh_assemble_body_force(fh_mesh* m, const double f[3], double* b_out);
+int fh_assemble_body_force_dev(fh_mesh* m, const double f[3], double* b_dev);
+/* poisson.rs:222-263 apply_dirichlet_bc / elasticity.rs:188-227
+ * apply_vector_dirichlet_bc: returns a new condensed matrix; b_in -> b_out
+ * (may alias).  Fixed rows become identity rows, couplings to fixed dofs are
+ * dropped (kept as explicit zeros on the device, dropped in the export).    */
+int fh_apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
+                       const uint64_t* dofs, const double* vals, fh_matrix** out,
+                       double* b_out);
+int fh_apply_dirichlet_dev(const fh_matrix* a, const double* b_in_dev, uint64_t nfix,
+                           const uint64_t* dofs, const double* vals, fh_matrix** out,
+                           double* b_out_dev);
+/* matrix shape, scalar nnz of the canonical CSC, block size */
+int fh_matrix_info(const fh_matrix* a, uint64_t* n, uint64_t* nnz, int* block);
+/* faer SparseColMat raw parts: col_ptr[n+1], row_idx[nnz], values[nnz]     */
+int fh_matrix_export_csc(const fh_matrix* a, uint64_t* col_ptr, uint64_t* row_idx, double* vals);
+int fh_matrix_spmv(const fh_matrix* a, const double* x, double* y, int order);
+int fh_matrix_spmv_dev(const fh_matrix* a, const double* x_dev, double* y_dev, int order);
+int fh_matrix_destroy(fh_matrix* a);
+
+/* ---------------------------------------------------------------- solve
+ * cg.rs:40-149 solve_cg with Jacobi preconditioning (same stopping rule:
+ * ||r_k||/||b|| < tol on the recursive residual; mixed modes additionally
+ * require the FP64 true residual ||b - A x||/||b|| < tol).  x0 may be NULL.
+ * Returns FH_ERR_NOT_CONVERGED / FH_ERR_BREAKDOWN like cg.rs (x_out is
+ * still written).                                                            */
+int fh_solve(const fh_matrix* a, const double* b, const double* x0, double tol,
+             uint64_t max_iter, int precision, double* x_out, fh_solve_stats* stats);
+int fh_solve_dev(const fh_matrix* a, const double* b_dev, const double* x0_dev, double tol,
+                 uint64_t max_iter, int precision, double* x_dev, fh_solve_stats* stats);
+
+/* ---------------------------------------------- drop-ins for gpu.rs entries */
+/* solve_cg (cg.rs:40-46) / solve_cg_gpu_or_cpu (gpu.rs:1542-1548) on faer's
+ * raw CSC parts; device -1 = best device                                  */
+int fh_solve_csc_cg(uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
+                    const double* vals, const double* b, const double* x0, double tol,
+                    uint64_t max_iter, int device, double* x_out, fh_solve_stats* stats);
+/* solve_csr_cg_gpu (gpu.rs:1145-1155): caller-provided inverse diagonal; x0
+ * is honoured when forming r0 (the reference ignores it, gpu.rs:1194)      */
+int fh_solve_csr_cg(uint64_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                    const double* vals, const double* inv_diag, const double* b,
+                    const double* x0, double rtol, uint64_t max_iter, int device,
+                    double* x_out, fh_solve_stats* stats);
+/* solve_bsr3_pcg_gpu (gpu.rs:775-786): 3x3 block CSR, 9 row-major values per
+ * block, 3x3 block-Jacobi with caller-provided dinv_flat [n_nodes][9]      */
+int fh_solve_bsr3_pcg(uint64_t n_nodes, const uint32_t* row_ptr, const uint32_t* cols,
+                      const double* vals_flat, const double* dinv_flat, const double* b,
+                      const double* x0, double rtol, uint64_t max_iter, int device,
+                      double* x_out, fh_solve_stats* stats);
+/* GpuBsr3Context::new / spmv / jacobi_smooth / device_name / nnz_blocks */
+int fh_bsr3_ctx_create(uint64_t n_nodes, const uint32_t* row_ptr, const uint32_t* cols,
+                       const double* vals_flat, const double* dinv_flat, int device,
+                       fh_bsr3_ctx** out);
+int fh_bsr3_ctx_spmv(fh_bsr3_ctx* c, const double* x, double* y);
+int fh_bsr3_ctx_jacobi_smooth(fh_bsr3_ctx* c, const double* r, double* z, uint64_t sweeps,
+                              double omega, int zero_initial_z);
+int fh_bsr3_ctx_device_name(const fh_bsr3_ctx* c, char* buf, size_t len);
+int fh_bsr3_ctx_nnz_blocks(const fh_bsr3_ctx* c, uint64_t* nnzb);
+int fh_bsr3_ctx_destroy(fh_bsr3_ctx* c);
+/* spmv_bsr3_multi_gpu (gpu.rs:662-669): contiguous block-row ranges over up
+ * to max_gpus devices; *n_used receives the number of devices used         */
+int fh_spmv_bsr3_multi(uint64_t n_nodes, const uint32_t* row_ptr, const uint32_t* cols,
+                       const double* vals_flat, const double* x, uint64_t max_gpus, double* y,
+                       int* n_used);
+
+/* -------------------------------------------------------------- drivers */
+/* poisson_solver.rs:26-59 / 99-132 solve_poisson_2d/3d: source at the
+ * degree-2 points fq [nc][nq], boundary values bvals at the boundary dofs
+ * (ascending, as fh_mesh_boundary_dofs returns them)                        */
+int fh_solve_poisson(fh_mesh* m, const double* fq, const double* bvals, double tol,
+                     uint64_t max_iter, int precision, double* u_out, fh_solve_stats* stats);
+/* elasticity_solver.rs:31-44 solve_elasticity_3d (assembly, body force,
+ * Dirichlet, CG); E_cell NULL or per-cell moduli on the host.  u_dev (device,
+ * 3*nv) receives the solution when non-NULL, u_out (host) otherwise.        */
+int fh_solve_elasticity_3d(fh_mesh* m, double E, double nu, const double* E_cell,
+                           const double f[3], uint64_t nfix, const uint64_t* dofs,
+                           const double* vals, double tol, uint64_t max_iter, int precision,
+                           double* u_out, double* u_dev, fh_solve_stats* stats);
+
+/* ------------------------------------------------- adjoint and stresses */
+/* out[c] = -w_c^T K0_c u_c with K0_c the E = 1 elasticity element matrix
+ * (elasticity.rs:133-136); with w = u this is dJ/dE_c of the compliance
+ * J = f^T u for K = sum_c E_c K0_c (new; no counterpart in the reference)   */
+int fh_cell_bilinear_k0_dev(fh_mesh* m, double nu, const double* w_dev, const double* u_dev,
+                            double* out_dev);
+int fh_cell_bilinear_k0(fh_mesh* m, double nu, const double* w, const double* u, double* out);
+/* elasticity.rs:230-320 compute_element_stresses: per cell stress[9],
+ * strain[9], von_mises, trace (20 doubles)                                  */
+int fh_element_stresses(fh_mesh* m, const double* u, double E, double nu, double* out);
+/* elasticity.rs:323-360 compute_vertex_von_mises */
+int fh_vertex_von_mises(fh_mesh* m, const double* vm_cell, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

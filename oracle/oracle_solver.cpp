@@ -1,0 +1,296 @@
+// oracle_solver.cpp — TEST INFRASTRUCTURE (checker only; see oracle.h).
+//
+// Jacobi-PCG and the CSC->CSR conversion, restated from fenics-rs:
+//   crates/fenics-solver/src/cg.rs:7-149   (spmv, dot, norm2, solve_cg)
+//   crates/fenics-solver/src/gpu.rs:1552-1583 (CSC -> CSR + inverse diagonal)
+// and the P2 tetrahedral dofmap / stiffness that the reference lacks:
+// vertices first, then edges in lexicographic (min, max) vertex order
+// (the vertices-then-edges convention of mass.rs:132-135); per-cell dofs in
+// P2Tetrahedron order (lagrange.rs:284-297: corners, then edges 01, 12, 02,
+// 03, 13, 23); K_e = sum_q (w_q |det J|) (g_i^q . g_j^q) with the degree-2
+// tetrahedron rule (quadrature.rs:157-184), exact for P2 x P2 gradients.
+#include <algorithm>
+#include <chrono>
+#include <thread>
+#include <cmath>
+#include <vector>
+
+#include "oracle_internal.hpp"
+
+using namespace orc;
+
+namespace {
+
+// cg.rs:27-29: sequential left-to-right sum of products.
+double dot(const double* a, const double* b, uint64_t n) {
+    double s = -0.0;
+    for (uint64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+// cg.rs:7-23: CSC column scatter that skips x[col] == 0.
+void spmv(const or_csc* A, const double* x, double* y) {
+    const uint64_t n = A->n;
+    for (uint64_t i = 0; i < n; ++i) y[i] = 0.0;
+    for (uint64_t c = 0; c < n; ++c) {
+        const double xv = x[c];
+        if (xv != 0.0)
+            for (uint64_t p = A->col_ptr[c]; p < A->col_ptr[c + 1]; ++p)
+                y[A->row_idx[p]] += A->vals[p] * xv;
+    }
+}
+
+// cg.rs:51-64: 1/d when |d| > 1e-15, else 1 (also when the diagonal is absent).
+std::vector<double> inv_diagonal(const or_csc* A) {
+    std::vector<double> d(A->n, 1.0);
+    for (uint64_t c = 0; c < A->n; ++c)
+        for (uint64_t p = A->col_ptr[c]; p < A->col_ptr[c + 1]; ++p)
+            if (A->row_idx[p] == c) {
+                if (std::fabs(A->vals[p]) > 1e-15) d[c] = 1.0 / A->vals[p];
+                break;
+            }
+    return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+void or_spmv(const or_csc* a, const double* x, double* y) { spmv(a, x, y); }
+
+// cg.rs:40-149, control flow and stopping rule kept exactly:
+// early exit when ||r0|| < tol or ||r0||/||b|| < tol; |pAp| < 1e-30 is a
+// breakdown (status 1); stop when the recursive ||r||/||b|| < tol; status 2
+// after max_iter.  *res is the absolute recursive residual norm.
+int or_solve_cg(const or_csc* A, const double* b, const double* x0, double tol,
+                uint64_t max_iter, double* x, uint64_t* iters, double* res) {
+    const uint64_t n = A->n;
+    const std::vector<double> inv = inv_diagonal(A);
+    for (uint64_t i = 0; i < n; ++i) x[i] = x0 ? x0[i] : 0.0;
+    std::vector<double> r(n), z(n), p(n), ap(n);
+    spmv(A, x, ap.data());
+    for (uint64_t i = 0; i < n; ++i) r[i] = b[i] - ap[i];
+    const double bnorm = std::sqrt(dot(b, b, n));
+    const double r0 = std::sqrt(dot(r.data(), r.data(), n));
+    *iters = 0;
+    *res = r0;
+    if (r0 < tol || (bnorm > 0.0 && r0 / bnorm < tol)) return 0;
+    for (uint64_t i = 0; i < n; ++i) z[i] = r[i] * inv[i];
+    p = z;
+    double rz_old = dot(r.data(), z.data(), n);
+    for (uint64_t it = 1; it <= max_iter; ++it) {
+        spmv(A, p.data(), ap.data());
+        const double pap = dot(p.data(), ap.data(), n);
+        *iters = it;
+        if (std::fabs(pap) < 1e-30) {
+            *res = std::sqrt(dot(r.data(), r.data(), n));
+            return 1;
+        }
+        const double alpha = rz_old / pap;
+        for (uint64_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * ap[i];
+        }
+        const double rn = std::sqrt(dot(r.data(), r.data(), n));
+        const double rel = bnorm > 0.0 ? rn / bnorm : rn;
+        *res = rn;
+        if (rel < tol) return 0;
+        for (uint64_t i = 0; i < n; ++i) z[i] = r[i] * inv[i];
+        const double rz_new = dot(r.data(), z.data(), n);
+        const double beta = rz_new / rz_old;
+        rz_old = rz_new;
+        for (uint64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return 2;
+}
+
+// The same loop body run for exactly `iters` iterations (no stopping test),
+// single-threaded like cg.rs; returns the wall time of the iterations only.
+double or_time_cg_iterations(const or_csc* A, const double* b, uint64_t iters) {
+    const uint64_t n = A->n;
+    const std::vector<double> inv = inv_diagonal(A);
+    std::vector<double> x(n, 0.0), r(b, b + n), z(n), p(n), ap(n);
+    for (uint64_t i = 0; i < n; ++i) z[i] = r[i] * inv[i];
+    p = z;
+    double rz_old = dot(r.data(), z.data(), n);
+    volatile double sink = 0.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t it = 1; it <= iters; ++it) {
+        spmv(A, p.data(), ap.data());
+        const double pap = dot(p.data(), ap.data(), n);
+        const double alpha = rz_old / pap;
+        for (uint64_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * ap[i];
+        }
+        sink = sink + std::sqrt(dot(r.data(), r.data(), n));
+        for (uint64_t i = 0; i < n; ++i) z[i] = r[i] * inv[i];
+        const double rz_new = dot(r.data(), z.data(), n);
+        const double beta = rz_new / rz_old;
+        rz_old = rz_new;
+        for (uint64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// gpu.rs:1552-1583: CSR rows filled in column order, inv_diag from |d| > 1e-15.
+void or_csc_to_csr(const or_csc* A, uint32_t* row_ptr, uint32_t* col_idx, double* vals,
+                   double* inv_diag) {
+    const uint64_t n = A->n;
+    std::vector<uint32_t> cnt(n, 0);
+    for (uint64_t i = 0; i < n; ++i) inv_diag[i] = 1.0;
+    for (uint64_t c = 0; c < n; ++c)
+        for (uint64_t p = A->col_ptr[c]; p < A->col_ptr[c + 1]; ++p) {
+            const uint32_t r = A->row_idx[p];
+            cnt[r]++;
+            if (r == c && std::fabs(A->vals[p]) > 1e-15) inv_diag[r] = 1.0 / A->vals[p];
+        }
+    row_ptr[0] = 0;
+    for (uint64_t i = 0; i < n; ++i) row_ptr[i + 1] = row_ptr[i] + cnt[i];
+    std::vector<uint32_t> cur(row_ptr, row_ptr + n);
+    for (uint64_t c = 0; c < n; ++c)
+        for (uint64_t p = A->col_ptr[c]; p < A->col_ptr[c + 1]; ++p) {
+            const uint32_t r = A->row_idx[p];
+            const uint32_t pos = cur[r]++;
+            col_idx[pos] = (uint32_t)c;
+            vals[pos] = A->vals[p];
+        }
+}
+
+// ---------------- P2 ----------------
+
+static const int P2_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+
+or_p2* or_p2_build(uint64_t nv, const double* verts, uint64_t nc, const uint64_t* cells) {
+    or_p2* p = new or_p2;
+    p->nv = nv;
+    p->nc = nc;
+    p->verts.assign(verts, verts + 3 * nv);
+    p->cells.assign(cells, cells + 4 * nc);
+    p->edges.reserve(6 * nc);
+    for (uint64_t c = 0; c < nc; ++c)
+        for (auto& e : P2_EDGE) {
+            const uint64_t a = cells[4 * c + e[0]], b = cells[4 * c + e[1]];
+            p->edges.push_back({std::min(a, b), std::max(a, b)});
+        }
+    std::sort(p->edges.begin(), p->edges.end());
+    p->edges.erase(std::unique(p->edges.begin(), p->edges.end()), p->edges.end());
+    p->ne = p->edges.size();
+    p->cell_dofs.resize(10 * nc);
+    for (uint64_t c = 0; c < nc; ++c) {
+        for (int a = 0; a < 4; ++a) p->cell_dofs[10 * c + a] = cells[4 * c + a];
+        for (int k = 0; k < 6; ++k) {
+            const uint64_t a = cells[4 * c + P2_EDGE[k][0]], b = cells[4 * c + P2_EDGE[k][1]];
+            const std::array<uint64_t, 2> key = {std::min(a, b), std::max(a, b)};
+            const auto it = std::lower_bound(p->edges.begin(), p->edges.end(), key);
+            p->cell_dofs[10 * c + 4 + k] = nv + (uint64_t)(it - p->edges.begin());
+        }
+    }
+    return p;
+}
+uint64_t or_p2_ndofs(const or_p2* p) { return p->nv + p->ne; }
+uint64_t or_p2_nedges(const or_p2* p) { return p->ne; }
+void or_p2_cell_dofs(const or_p2* p, uint64_t* out) {
+    std::copy(p->cell_dofs.begin(), p->cell_dofs.end(), out);
+}
+void or_p2_dof_coords(const or_p2* p, double* out) {
+    std::copy(p->verts.begin(), p->verts.end(), out);
+    for (uint64_t e = 0; e < p->ne; ++e)
+        for (int d = 0; d < 3; ++d)
+            out[3 * (p->nv + e) + d] =
+                (p->verts[3 * p->edges[e][0] + d] + p->verts[3 * p->edges[e][1] + d]) * 0.5;
+}
+// Boundary P2 dofs: vertices and edges of the boundary facets (mesh.rs:145-210).
+int or_p2_boundary_dofs(const or_p2* p, uint64_t* out, uint64_t* n_out) {
+    uint64_t nf = 0;
+    or_boundary_facets_3d(p->nc, p->cells.data(), nullptr, &nf);
+    std::vector<uint64_t> f(3 * nf);
+    or_boundary_facets_3d(p->nc, p->cells.data(), f.data(), &nf);
+    std::vector<uint64_t> d;
+    d.reserve(6 * nf);
+    for (uint64_t i = 0; i < nf; ++i) {
+        const uint64_t* t = &f[3 * i];
+        for (int a = 0; a < 3; ++a) d.push_back(t[a]);
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto& pr : pairs) {
+            const std::array<uint64_t, 2> key = {std::min(t[pr[0]], t[pr[1]]),
+                                                 std::max(t[pr[0]], t[pr[1]])};
+            const auto it = std::lower_bound(p->edges.begin(), p->edges.end(), key);
+            d.push_back(p->nv + (uint64_t)(it - p->edges.begin()));
+        }
+    }
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    if (out) std::copy(d.begin(), d.end(), out);
+    *n_out = d.size();
+    return 0;
+}
+
+or_csc* or_assemble_p2_stiffness_3d(const or_p2* p, int nthreads) {
+    const Rule r = rule(3, 2);
+    double G[4][10][3];
+    for (int q = 0; q < 4; ++q) p2_tet_grads(r.pts[q].data(), G[q]);
+    const int nth = nthreads_or_default(nthreads);
+    std::vector<std::vector<Trip>> parts(nth);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+        th.emplace_back([&, t] {
+            const uint64_t a = p->nc * t / nth, b = p->nc * (t + 1) / nth;
+            auto& out = parts[t];
+            out.reserve((b - a) * 100);
+            for (uint64_t c = a; c < b; ++c) {
+                const uint64_t* cv = &p->cells[4 * c];
+                const double* V = p->verts.data();
+                Map3 m;
+                if (!affine3(V + 3 * cv[0], V + 3 * cv[1], V + 3 * cv[2], V + 3 * cv[3], m))
+                    continue;
+                const double detj = std::fabs(m.det);
+                double K[10][10];
+                for (int i = 0; i < 10; ++i)
+                    for (int j = 0; j < 10; ++j) K[i][j] = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    double g[10][3];
+                    for (int i = 0; i < 10; ++i) grad3(m, G[q][i], g[i]);
+                    const double s = r.w[q] * detj;
+                    for (int i = 0; i < 10; ++i)
+                        for (int j = 0; j < 10; ++j)
+                            K[i][j] += s * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]);
+                }
+                const uint64_t* d = &p->cell_dofs[10 * c];
+                for (int i = 0; i < 10; ++i)
+                    for (int j = 0; j < 10; ++j)
+                        out.push_back({(uint32_t)d[i], (uint32_t)d[j], K[i][j]});
+            }
+        });
+    for (auto& x : th) x.join();
+    std::vector<Trip> all;
+    for (auto& pp : parts) all.insert(all.end(), pp.begin(), pp.end());
+    return csc_from_trips(p->nv + p->ne, all);
+}
+
+// poisson.rs:168-218 restated for P2 with a per-cell constant source:
+// local_b[i] += w * |det| * f_c * phi_i(xi_q), scattered in cell order.
+int or_assemble_p2_rhs_cellconst(const or_p2* p, const double* f_cell, double* b) {
+    const Rule r = rule(3, 2);
+    const uint64_t nd = p->nv + p->ne;
+    for (uint64_t i = 0; i < nd; ++i) b[i] = 0.0;
+    for (uint64_t c = 0; c < p->nc; ++c) {
+        const uint64_t* cv = &p->cells[4 * c];
+        const double* V = p->verts.data();
+        Map3 m;
+        if (!affine3(V + 3 * cv[0], V + 3 * cv[1], V + 3 * cv[2], V + 3 * cv[3], m)) continue;
+        const double detj = std::fabs(m.det);
+        double lb[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 4; ++q) {
+            double phi[10];
+            p2_tet_basis(r.pts[q].data(), phi);
+            for (int i = 0; i < 10; ++i) lb[i] += r.w[q] * detj * f_cell[c] * phi[i];
+        }
+        for (int i = 0; i < 10; ++i) b[p->cell_dofs[10 * c + i]] += lb[i];
+    }
+    return 0;
+}
+void or_p2_free(or_p2* p) { delete p; }
+
+}  // extern "C"

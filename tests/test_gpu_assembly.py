@@ -1,0 +1,190 @@
+"""GPU parity of the assembly path against the CPU oracle.
+
+Integer data (meshes, boundary dofs, CSC patterns, P2 dofmaps) must be
+identical; FP64 values must be bit-identical (the GPU sums every slot's
+duplicates in ascending cell order with the oracle's operation order and no
+FMA).  Full-size configurations are checked through exact counts and
+size-independent invariants."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fenics_hip as fh
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def same_csc(gpu_mat, ora_mat):
+    cg, rg, vg = gpu_mat.export_csc()
+    co, ro, vo = ora_mat.export()
+    assert np.array_equal(cg, co), "col_ptr differs"
+    assert np.array_equal(rg, ro), "row_idx differs"
+    assert np.array_equal(bits(vg), bits(vo)), \
+        f"values differ (max |d| {np.abs(vg - vo).max():.3e})"
+
+
+def gpu_mesh_equals_oracle(gm, om):
+    assert np.array_equal(bits(gm.vertices), bits(om.vertices))
+    assert np.array_equal(gm.cells, om.cells)
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (4, 4), (7, 3), (32, 32)])
+def test_unit_square_bitwise(nx, ny):
+    gm, om = fh.unit_square(nx, ny), O.unit_square(nx, ny)
+    gpu_mesh_equals_oracle(gm, om)
+    assert np.array_equal(gm.boundary_dofs(), O.boundary_dofs(om))
+    same_csc(fh.assemble_stiffness_2d(gm), O.assemble_stiffness(om))
+    same_csc(fh.assemble_mass(gm), O.assemble_mass_2d(om))
+
+
+@pytest.mark.parametrize("dims", [(1.0, 1.0, 1.0, 3, 3, 3), (10.0, 2.0, 2.0, 8, 3, 2)])
+def test_box_beam_bitwise(dims):
+    gm, om = fh.box_beam(*dims), O.box_beam(*dims)
+    gpu_mesh_equals_oracle(gm, om)
+    assert np.array_equal(gm.boundary_dofs(), O.boundary_dofs(om))
+    same_csc(fh.assemble_stiffness_3d(gm), O.assemble_stiffness(om))
+
+
+def test_golden_p1_fixture():
+    g = np.load(os.path.join(GOLDEN, "p1_unit_square_4x4.npz"))
+    cp, ri, va = fh.assemble_stiffness(fh.unit_square(4, 4)).export_csc()
+    assert np.array_equal(cp, g["col_ptr"]) and np.array_equal(ri, g["row_idx"])
+    assert np.array_equal(bits(va), bits(g["values"]))
+
+
+def test_golden_elasticity_fixture():
+    g = np.load(os.path.join(GOLDEN, "elasticity_box_beam_2x1x1.npz"))
+    m = fh.box_beam(1.0, 0.1, 0.1, 2, 1, 1)
+    cp, ri, va = fh.assemble_elasticity_stiffness_3d(m, fh.ElasticMaterial(1e7, 0.3)).export_csc()
+    assert np.array_equal(cp, g["col_ptr"]) and np.array_equal(ri, g["row_idx"])
+    assert np.array_equal(bits(va), bits(g["values"]))
+
+
+def test_mesh_from_arrays_roundtrip():
+    om = O.box_beam(2.0, 1.0, 1.0, 3, 2, 2)
+    gm = fh.Mesh.from_arrays(om.vertices, om.cells)
+    gpu_mesh_equals_oracle(gm, om)
+    same_csc(fh.assemble_stiffness(gm), O.assemble_stiffness(om))
+
+
+def test_quadrature_points_and_rhs_bitwise():
+    for gm, om in [(fh.unit_square(9, 5), O.unit_square(9, 5)),
+                   (fh.unit_cube(3, 2, 4), O.unit_cube(3, 2, 4))]:
+        xg, xo = gm.quad_points(2), O.quad_points(om, 2)
+        assert np.array_equal(bits(xg), bits(xo))
+        fq = np.sin(3 * xo[..., 0]) + xo[..., 1] ** 2
+        assert np.array_equal(bits(fh.assemble_rhs(gm, fq)), bits(O.assemble_rhs(om, fq)))
+
+
+@pytest.mark.parametrize("E", [210e9, "cells"])
+def test_elasticity_bitwise(E):
+    dims = (1.0, 0.1, 0.1, 6, 2, 2)
+    gm, om = fh.box_beam(*dims), O.box_beam(*dims)
+    if E == "cells":
+        Ec = 210e9 * np.exp(0.3 * np.random.default_rng(7).standard_normal(om.nc))
+        g = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(210e9, 0.3), E_cell=Ec)
+        o = O.assemble_elasticity(om, Ec, 0.3)
+    else:
+        g = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(E, 0.3))
+        o = O.assemble_elasticity(om, E, 0.3)
+    same_csc(g, o)
+    f = [0.3, -1.0, -7.85e4]
+    assert np.array_equal(bits(fh.assemble_elasticity_body_force_3d(gm, f)),
+                          bits(O.assemble_body_force(om, f)))
+
+
+def test_dirichlet_scalar_bitwise():
+    gm, om = fh.unit_square(10, 8), O.unit_square(10, 8)
+    b = np.random.default_rng(2).standard_normal(om.nv)
+    bd = O.boundary_dofs(om)
+    g = np.linspace(-1.0, 2.0, len(bd))
+    A1, b1 = fh.apply_dirichlet_bc(fh.assemble_stiffness(gm), b, fh.DirichletBC(bd, g))
+    A2, b2 = O.apply_dirichlet_bc(O.assemble_stiffness(om), b, bd, g)
+    same_csc(A1, A2)
+    assert np.array_equal(bits(b1), bits(b2))
+
+
+def test_dirichlet_vector_partial_and_duplicate_bitwise():
+    dims = (1.0, 0.2, 0.2, 4, 2, 2)
+    gm, om = fh.box_beam(*dims), O.box_beam(*dims)
+    K1 = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(1e6, 0.25))
+    K2 = O.assemble_elasticity(om, 1e6, 0.25)
+    b = fh.assemble_elasticity_body_force_3d(gm, [0, 0, -1.0])
+    # x-clamp plus a few single-component constraints with nonzero values and
+    # one dof listed twice (reference: last value wins, diagonal 1.0 + 1.0)
+    pres = [(d, 0.0) for d in O.clamp_dofs_x0(om)] + [(31, 1e-3), (47, -2e-3), (31, 5e-4)]
+    A1, b1 = fh.apply_vector_dirichlet_bc(K1, b, fh.VectorDirichletBC(pres))
+    A2, b2 = O.apply_dirichlet_bc(K2, b, [d for d, _ in pres], [v for _, v in pres])
+    same_csc(A1, A2)
+    assert np.array_equal(bits(b1), bits(b2))
+
+
+def test_p2_dofmap_stiffness_rhs_bitwise():
+    gm, om = fh.unit_cube(3, 3, 2), O.unit_cube(3, 3, 2)
+    p2 = O.P2(om)
+    assert gm.p2_info() == (p2.ndofs, p2.nedges)
+    assert np.array_equal(gm.p2_cell_dofs(), p2.cell_dofs())
+    assert np.array_equal(gm.p2_boundary_dofs(), p2.boundary_dofs())
+    same_csc(fh.assemble_p2_stiffness(gm, 0), p2.stiffness())
+    fc = np.random.default_rng(9).random(om.nc)
+    assert np.array_equal(bits(fh.assemble_p2_rhs_cellconst(gm, fc)), bits(p2.rhs_cellconst(fc)))
+
+
+# ---------------------------------------------------------------- full-size configurations
+COUNTS = json.load(open(os.path.join(GOLDEN, "counts.json")))
+
+
+def test_c0_full_size_bitwise():
+    gm, om = fh.unit_square(128, 128), O.unit_square(128, 128)
+    A = fh.assemble_stiffness(gm)
+    same_csc(A, O.assemble_stiffness(om))
+    bd = gm.boundary_dofs()
+    Ab, _ = fh.apply_dirichlet_bc(A, np.zeros(gm.nv), fh.DirichletBC(bd, np.zeros(len(bd))))
+    assert Ab.nnz() == COUNTS["C0"]["nnz_bc"]
+
+
+def test_c3_full_size_bitwise():
+    dims = (10.0, 2.0, 2.0, 100, 20, 20)
+    gm, om = fh.box_beam(*dims), O.box_beam(*dims)
+    K = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(210e9, 0.3))
+    same_csc(K, O.assemble_elasticity(om, 210e9, 0.3))
+
+
+def test_c1_full_size_counts_and_invariants():
+    gm = fh.box_beam(10.0, 2.0, 2.0, 200, 40, 40)
+    c = COUNTS["C1"]
+    assert 3 * gm.nv == c["n"] and gm.nc == c["cells"]
+    K = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(210e9, 0.3))
+    assert K.nnz() == c["nnz_raw"]
+    # rigid translations are in the kernel before the boundary conditions
+    for a in range(3):
+        t = np.zeros((gm.nv, 3))
+        t[:, a] = 1.0
+        r = K.spmv(t.reshape(-1))
+        assert np.abs(r).max() < 1e-6 * 210e9 * 0.05
+    bc = fh.clamp_dofs_x0(gm.vertices)
+    assert len(bc.prescribed_dofs) == c["n_fixed"]
+    b = fh.assemble_elasticity_body_force_3d(gm, [0, 0, -7.85e4])
+    assert abs(b.sum() - (-7.85e4 * 40.0)) < 1e-9 * 7.85e4 * 40.0
+    Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
+    assert Kb.nnz() == c["nnz_bc"]
+
+
+def test_c2_full_size_p2_counts():
+    gm = fh.unit_cube(64, 64, 64)
+    c = COUNTS["C2"]
+    nd, ne = gm.p2_info()
+    assert (nd, ne) == (c["n"], c["n_edges"])
+    assert len(gm.p2_boundary_dofs()) == c["n_fixed"]
+    K = fh.assemble_p2_stiffness(gm, 0)
+    assert K.nnz() == c["nnz_raw"]
+    r = K.spmv(np.ones(nd))
+    assert np.abs(r).max() < 1e-12

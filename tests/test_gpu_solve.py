@@ -1,0 +1,257 @@
+"""GPU parity of the solvers against the CPU oracle and the reference's own
+tests (cg.rs:156-169, poisson_tests.rs, elasticity_tests.rs,
+ab_benchmarks.rs, gpu.rs:1609-1769).
+
+Tolerances: FP64 CG differs from the oracle's only in the summation order of
+its dot products, so solutions agree to ~kappa * 1e-16 relative (checked at
+1e-8 for tol 1e-12 solves); every solve's true FP64 residual is checked
+against its tolerance; mixed-precision solves must reach the same true
+residual and agree with the FP64 solution to 1e-7 relative."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fenics_hip as fh
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+PI = np.pi
+
+
+def src2d(x):
+    return 2 * PI * PI * np.sin(PI * x[0]) * np.sin(PI * x[1])
+
+
+def exact2d(x):
+    return np.sin(PI * x[0]) * np.sin(PI * x[1])
+
+
+def test_small_spd_system():  # cg.rs:156-169
+    res = fh.solve_cg_gpu_or_cpu([0, 2, 4], [0, 1, 0, 1], [4.0, -1.0, -1.0, 4.0], [3.0, 3.0],
+                                 None, 1e-10, 100)
+    assert res.iterations <= 2 and res.stats.final_residual < 1e-10
+    assert np.allclose(res.solution, 1.0, atol=1e-8)
+
+
+def c0_system():
+    om = O.unit_square(128, 128)
+    A = O.assemble_stiffness(om)
+    b = O.assemble_rhs(om, src2d(np.moveaxis(O.quad_points(om, 2), -1, 0)))
+    bd = O.boundary_dofs(om)
+    return O.apply_dirichlet_bc(A, b, bd, np.zeros(len(bd)))
+
+
+def test_c0_csc_drop_in_vs_oracle():
+    Ab, bb = c0_system()
+    st, xo, it_o, res_o = O.solve_cg(Ab, bb, None, 1e-12, 5000)
+    cp, ri, va = Ab.export()
+    r = fh.solve_cg_gpu_or_cpu(cp, ri, va, bb, None, 1e-12, 5000)
+    assert abs(r.iterations - it_o) <= 2
+    assert np.linalg.norm(r.solution - xo) / np.linalg.norm(xo) < 1e-8
+    assert r.stats.true_rel_residual < 2e-12
+
+
+def test_ordered_spmv_is_bitwise_reference_spmv():
+    Ab, _ = c0_system()
+    cp, ri, va = Ab.export()
+    x = np.random.default_rng(0).standard_normal(Ab.n)
+    x[::5] = 0.0
+    gm = fh.unit_square(128, 128)
+    A = fh.assemble_stiffness(gm)
+    bd = gm.boundary_dofs()
+    Ag, _ = fh.apply_dirichlet_bc(A, np.zeros(gm.nv), fh.DirichletBC(bd, np.zeros(len(bd))))
+    y = Ag.spmv(x, fh.ORDER_ORDERED)
+    assert np.array_equal(y.view(np.uint64), Ab.spmv(x).view(np.uint64))
+    om = O.box_beam(1.0, 0.2, 0.2, 5, 2, 2)
+    K = fh.assemble_elasticity_stiffness_3d(fh.box_beam(1.0, 0.2, 0.2, 5, 2, 2),
+                                            fh.ElasticMaterial(3e3, 0.3))
+    xe = np.random.default_rng(1).standard_normal(3 * om.nv)
+    ye = O.assemble_elasticity(om, 3e3, 0.3).spmv(xe)
+    assert np.array_equal(K.spmv(xe, fh.ORDER_ORDERED).view(np.uint64), ye.view(np.uint64))
+
+
+def test_poisson_2d_manufactured_rate():  # poisson_tests.rs:6-52
+    errs = []
+    for n in (8, 16, 32):
+        m = fh.unit_square(n, n)
+        sol = fh.solve_poisson_2d(m, src2d, exact2d, 1e-10, 500)
+        om = O.unit_square(n, n)
+        errs.append(O.l2_error(om, sol.u, exact2d(np.moveaxis(O.quad_points(om, 3), -1, 0))))
+    assert 3.8 < errs[0] / errs[1] < 4.2 and 3.8 < errs[1] / errs[2] < 4.2
+
+
+def test_poisson_3d_manufactured():  # poisson_tests.rs:55-73
+    ex = lambda x: np.sin(PI * x[0]) * np.sin(PI * x[1]) * np.sin(PI * x[2])
+    src = lambda x: 3 * PI * PI * ex(x)
+    m = fh.unit_cube(8, 8, 8)
+    sol = fh.solve_poisson_3d(m, src, ex, 1e-10, 1000)
+    om = O.unit_cube(8, 8, 8)
+    assert O.l2_error(om, sol.u, ex(np.moveaxis(O.quad_points(om, 3), -1, 0))) < 0.05
+
+
+def test_ab_poisson_2d_vs_dolfinx():  # ab_benchmarks.rs:13-48
+    ref = json.load(open(os.path.join(GOLDEN, "dolfinx_reference.json")))["poisson_2d"]
+    m = fh.unit_square(16, 16)
+    sol = fh.solve_poisson_2d(m, src2d, exact2d, 1e-12, 1000)
+    om = O.unit_square(16, 16)
+    l2 = O.l2_error(om, sol.u, exact2d(np.moveaxis(O.quad_points(om, 3), -1, 0)))
+    assert abs(l2 - ref["l2_error"]) / ref["l2_error"] < 0.05
+    assert abs(sol.u.max() - ref["max_value"]) / ref["max_value"] < 1e-3
+
+
+def tip_mean(m, u, comp, L=1.0):
+    tip = np.abs(m.vertices[:, 0] - L) < 1e-6
+    return u.reshape(-1, 3)[tip, comp].mean()
+
+
+def test_ab_cantilever_vs_dolfinx():  # ab_benchmarks.rs:51-100
+    ref = json.load(open(os.path.join(GOLDEN, "dolfinx_reference.json")))["elasticity_3d"]
+    m = fh.box_beam(1.0, 0.1, 0.1, 10, 2, 2)
+    sol = fh.solve_elasticity_3d(m, fh.ElasticMaterial(1e7, 0.3), [0, -1000.0, 0],
+                                 fh.clamp_dofs_x0(m.vertices), 1e-8, 1000)
+    assert abs(tip_mean(m, sol.displacements, 1) - ref["tip_uy"]) / abs(ref["tip_uy"]) < 0.01
+
+
+def test_axial_bar_extension():  # elasticity_tests.rs:6-49
+    m = fh.box_beam(1.0, 0.1, 0.1, 10, 2, 2)
+    sol = fh.solve_elasticity_3d(m, fh.ElasticMaterial(1.0e6, 0.0), [1000.0, 0, 0],
+                                 fh.clamp_dofs_x0(m.vertices), 1e-10, 1000)
+    assert abs(tip_mean(m, sol.displacements, 0) - 5e-4) / 5e-4 < 0.03
+
+
+def test_cantilever_beam_bending_vs_oracle():  # elasticity_tests.rs:52-126
+    m = fh.box_beam(1.0, 0.1, 0.1, 20, 3, 3)
+    sol = fh.solve_elasticity_3d(m, fh.ElasticMaterial(1.0e6, 0.3), [0, 0, -1000.0],
+                                 fh.clamp_dofs_x0(m.vertices), 1e-9, 2000)
+    uz = tip_mean(m, sol.displacements, 2)
+    assert abs(uz - (-0.0890)) / 0.0890 < 0.06 and uz < 0
+    om = O.box_beam(1.0, 0.1, 0.1, 20, 3, 3)
+    K = O.assemble_elasticity(om, 1.0e6, 0.3)
+    b = O.assemble_body_force(om, [0, 0, -1000.0])
+    fx = O.clamp_dofs_x0(om)
+    Kb, bb = O.apply_dirichlet_bc(K, b, fx, np.zeros(len(fx)))
+    st, uo, it, res = O.solve_cg(Kb, bb, None, 1e-9, 2000)
+    assert abs(sol.iterations - it) <= 3
+    assert np.linalg.norm(sol.displacements - uo) / np.linalg.norm(uo) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [fh.MIXED, fh.MIXED_IR])
+def test_mixed_precision_reaches_fp64_accuracy(prec):
+    m = fh.box_beam(10.0, 2.0, 2.0, 50, 10, 10)
+    mat = fh.ElasticMaterial(210e9, 0.3)
+    bc = fh.clamp_dofs_x0(m.vertices)
+    f = [0, 0, -7.85e4]
+    s64 = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 20000, compute_stresses=False)
+    smx = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 40000, precision=prec,
+                                 compute_stresses=False)
+    assert s64.stats.true_rel_residual < 1.5e-10
+    assert smx.stats.true_rel_residual < 1e-10
+    u64, umx = s64.displacements, smx.displacements
+    assert np.linalg.norm(umx - u64) / np.linalg.norm(u64) < 1e-7
+    if prec == fh.MIXED:  # residual replacement keeps the Krylov space
+        assert smx.iterations < 1.3 * s64.iterations
+
+
+def test_error_paths():
+    Ab, bb = c0_system()
+    cp, ri, va = Ab.export()
+    with pytest.raises(fh.FenicsError) as e:
+        fh.solve_cg_gpu_or_cpu(cp, ri, va, bb, None, 1e-12, 3)
+    assert e.value.code == fh.FH_ERR_NOT_CONVERGED
+    r = fh.solve_cg_gpu_or_cpu(cp, ri, va, np.zeros(len(bb)), None, 1e-12, 10)
+    assert r.iterations == 0 and not r.solution.any()
+    with pytest.raises(fh.FenicsError) as e:
+        fh.solve_cg_gpu_or_cpu([0, 1, 2], [0, 1], [0.0, 0.0], [1.0, 1.0], None, 1e-10, 10)
+    assert e.value.code == fh.FH_ERR_BREAKDOWN
+    with pytest.raises(fh.FenicsError) as e:
+        fh.solve_cg_gpu_or_cpu([0, 1, 2], [0, 5], [1.0, 1.0], [1.0, 1.0], None, 1e-10, 10)
+    assert e.value.code == fh.FH_ERR_BAD_INPUT
+    with pytest.raises(ValueError):
+        fh.ElasticMaterial(1.0, 0.5)
+
+
+def laplace7(nx):
+    """gpu.rs:1614-1643: 3D 7-point finite-difference Laplacian."""
+    n = nx ** 3
+    rows, cols, vals = [], [], []
+    for z in range(nx):
+        for y in range(nx):
+            for x in range(nx):
+                i = (z * nx + y) * nx + x
+                rows.append(i), cols.append(i), vals.append(6.0)
+                for d, ok in ((-1, x > 0), (1, x + 1 < nx), (-nx, y > 0), (nx, y + 1 < nx),
+                              (-nx * nx, z > 0), (nx * nx, z + 1 < nx)):
+                    if ok:
+                        rows.append(i), cols.append(i + d), vals.append(-1.0)
+    return O.csc_from_triplets(n, rows, cols, vals)
+
+
+def test_csr_cg_vs_cpu():  # gpu.rs:1609-1663
+    A = laplace7(20)
+    b = np.ones(A.n)
+    st, xo, it, res = O.solve_cg(A, b, None, 1e-9, 1000)
+    rp, ci, va, inv = A.to_csr()
+    r = fh.solve_csr_cg_gpu(rp, ci, va, inv, b, None, 1e-9, 1000)
+    for i in range(0, A.n, 137):
+        assert abs(r.solution[i] - xo[i]) <= 1e-5 * max(1.0, abs(xo[i]))
+    cp, ri, vv = A.export()
+    r2 = fh.solve_cg_gpu_or_cpu(cp, ri, vv, b, None, 1e-9, 1000)
+    assert np.allclose(r2.solution, xo, rtol=1e-6, atol=1e-8)
+    # x0 is honoured when forming r0 (gpu.rs:1194 ignores it)
+    r3 = fh.solve_csr_cg_gpu(rp, ci, va, inv, b, xo, 1e-9, 1000)
+    assert r3.iterations <= 1
+
+
+def bsr3_synthetic(nx=15):
+    """gpu.rs:1671-1741: synthetic SPD 3x3 block-CSR on a 15^3 grid."""
+    n_nodes = nx ** 3
+    diag = np.array([18.0, 1.0, 0.5, 1.0, 18.0, 0.5, 0.5, 0.5, 18.0])
+    off = np.array([-2.0, 0, 0, 0, -2.0, 0, 0, 0, -2.0])
+    inv = np.linalg.inv(diag.reshape(3, 3)).reshape(-1)
+    row_ptr, cols, vals = [0], [], []
+    for z in range(nx):
+        for y in range(nx):
+            for x in range(nx):
+                i = (z * nx + y) * nx + x
+                nb = [i] + [i + d for d, ok in ((-1, x > 0), (1, x + 1 < nx), (-nx, y > 0),
+                                                (nx, y + 1 < nx), (-nx * nx, z > 0),
+                                                (nx * nx, z + 1 < nx)) if ok]
+                for j in sorted(nb):
+                    cols.append(j)
+                    vals.append(diag if j == i else off)
+                row_ptr.append(len(cols))
+    return n_nodes, np.array(row_ptr), np.array(cols), np.concatenate(vals), np.tile(inv, n_nodes)
+
+
+def test_bsr3_context_multi_and_pcg():  # gpu.rs:1665-1768
+    n_nodes, rp, cc, va, dinv = bsr3_synthetic()
+    x = (np.arange(3 * n_nodes) % 17) * 0.1
+    ctx = fh.GpuBsr3Context(n_nodes, rp, cc, va, dinv)
+    y1 = ctx.spmv(x)
+    y2, used = fh.spmv_bsr3_multi_gpu(n_nodes, rp, cc, va, x, 3)
+    assert len(used) >= 1 and np.allclose(y1, y2, atol=1e-12, rtol=0)
+    # reference product on the host
+    yr = np.zeros(3 * n_nodes)
+    for r in range(n_nodes):
+        for p in range(rp[r], rp[r + 1]):
+            yr[3 * r:3 * r + 3] += va[9 * p:9 * p + 9].reshape(3, 3) @ x[3 * cc[p]:3 * cc[p] + 3]
+    assert np.allclose(y1, yr, rtol=1e-14, atol=1e-12)
+    assert ctx.nnz_blocks() == len(cc) and ctx.device_name()
+    res = fh.solve_bsr3_pcg_gpu(n_nodes, rp, cc, va, dinv, np.ones(3 * n_nodes), None, 1e-9, 500)
+    assert res.stats.rel_residual < 1e-9 and res.stats.true_rel_residual < 1e-8
+    # block-Jacobi smoothing reduces the residual of A z = r
+    r = np.ones(3 * n_nodes)
+    z = ctx.jacobi_smooth(r, np.zeros(3 * n_nodes), 6, 0.8, True)
+    assert np.linalg.norm(r - ctx.spmv(z)) < 0.5 * np.linalg.norm(r)
+    z1 = ctx.jacobi_smooth(r, np.zeros(3 * n_nodes), 1, 0.8, True)
+    assert np.allclose(z1, (dinv.reshape(-1, 3, 3) @ r.reshape(-1, 3, 1)).reshape(-1))
+
+
+def test_solver_determinism():
+    m = fh.unit_square(64, 64)
+    a = fh.solve_poisson_2d(m, src2d, exact2d, 1e-12, 2000)
+    b = fh.solve_poisson_2d(m, src2d, exact2d, 1e-12, 2000)
+    assert a.iterations == b.iterations and np.array_equal(a.u, b.u)
