@@ -726,15 +726,16 @@ int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out) {
         hipStream_t s = stream_of(m->dev);
         MeshP2& p2 = mesh_p2(m);
         std::unique_ptr<fh_matrix> a(new_matrix(m->dev, p2.pat, 1));
-        if (method == 0) {
+        FH_CHECK(method >= 0 && method <= 2, "P2 method must be 0 (VALU FP64), 1 (MFMA FP32) "
+                                             "or 2 (both: FP64 values + MFMA FP32 copy)");
+        if (method != 1) {
             hipLaunchKernelGGL(k_asm_p2_valu, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s,
                                m->verts.p, m->cells.p, p2.adj->ptr.p, p2.adj->ent.p,
                                p2.adj->slots.p, p2.pat->slice_off.p, p2.pat->row_len.p,
                                p2.adj->ndofs, p2_table(), a->vals.p);
             FH_LAUNCH_CHECK("asm_p2_valu");
-        } else {
-            throw Error(FH_ERR_UNSUPPORTED, "P2 MFMA assembly not built");
         }
+        if (method != 0) assemble_p2_mfma(m, a.get(), method == 1, s);
         FH_HIP(hipStreamSynchronize(s));
         *out = a.release();
     });

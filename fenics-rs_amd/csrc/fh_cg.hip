@@ -499,6 +499,15 @@ void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStre
     FH_LAUNCH_CHECK("spmv");
 }
 
+void inv_diag_f64(const fh_matrix* a, double* out, hipStream_t s) {
+    const SellA<double> A = sell_of(a, a->vals.p);
+    if (a->bs == 1)
+        hipLaunchKernelGGL((k_inv_diag<1, double>), dim3(cdiv(a->n, 256)), dim3(256), 0, s, A, out);
+    else
+        hipLaunchKernelGGL((k_inv_diag<3, double>), dim3(cdiv(a->n, 256)), dim3(256), 0, s, A, out);
+    FH_LAUNCH_CHECK("inv_diag");
+}
+
 void spmv_f32(const fh_matrix* a, const float* x, float* y, hipStream_t s) {
     const SellA<float> A = sell_of(a, a->vals32.p);
     const unsigned g = spmv_grid(a);

@@ -107,10 +107,12 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
 fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_ptr,
                            const uint32_t* cols, const double* vals);
 void ensure_vals32(fh_matrix* a);
+void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s);
 uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
 
 // SpMV (fh_cg.hip)
 void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStream_t s);
+void inv_diag_f64(const fh_matrix* a, double* out, hipStream_t s);
 
 // Solve (fh_cg.hip)
 struct SolveArgs {

@@ -729,6 +729,26 @@ def cell_bilinear_k0(mesh: Mesh, nu, w, u) -> np.ndarray:
     return out
 
 
+def batched_poisson_block_cg(sizes, f_nodal, tol=1e-10, max_iter=20000, precision=FP64,
+                             device=-1):
+    """Many unit_square(N_i, N_i) Poisson problems with 4 right-hand sides each
+    (b = M f from nodal loads f: list of [(N_i+1)^2, 4] arrays).  Returns
+    (list of solutions, iterations [n_inst, 4], SolveStats)."""
+    sizes = _u64(sizes)
+    f = _f64(np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1, 4) for a in f_nodal]))
+    u = np.empty_like(f)
+    its = np.empty((len(sizes), 4), np.uint64)
+    st = SolveStats()
+    _check(lib().fh_batched_poisson_block_cg(len(sizes), _p(sizes), 4, _p(f), tol, max_iter,
+                                             precision, device, _p(u), _p(its), C.byref(st)))
+    out, o = [], 0
+    for N in sizes:
+        k = (int(N) + 1) ** 2
+        out.append(u[o:o + k])
+        o += k
+    return out, its, st
+
+
 def clamp_dofs_x0(vertices, tol=1e-6):
     """elasticity_tests.rs:94-101: all three dofs of the vertices with x < 1e-6."""
     idx = np.nonzero(np.asarray(vertices)[:, 0] < tol)[0]

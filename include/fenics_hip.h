@@ -228,6 +228,19 @@ int fh_element_stresses(fh_mesh* m, const double* u, double E, double nu, double
 /* elasticity.rs:323-360 compute_vertex_von_mises */
 int fh_vertex_von_mises(fh_mesh* m, const double* vm_cell, double* out);
 
+/* ------------------------------------------------ batched multi-RHS solve
+ * New (the reference solves one system per solve_poisson_2d call,
+ * poisson_solver.rs:26): n_inst independent unit_square(N_i, N_i) Poisson P1
+ * problems (generators.rs:24-55, u = 0 on the boundary) with nrhs (= 4)
+ * right-hand sides each, b = M f (M the P1 mass matrix, mass.rs:10-44; f the
+ * nodal loads f_nodal[sum_i (N_i+1)^2][nrhs]).  Every column is solved by
+ * Jacobi-PCG to ||r||/||b|| < tol (true FP64 residual in the mixed modes);
+ * u_out has f_nodal's layout, iters_out[n_inst][nrhs] the per-column counts. */
+int fh_batched_poisson_block_cg(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
+                                const double* f_nodal, double tol, uint64_t max_iter,
+                                int precision, int device, double* u_out,
+                                uint64_t* iters_out, fh_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
