@@ -82,13 +82,13 @@ def main():
         return st
 
     log(f"mesh: {mesh.nv} vertices, {mesh.nc} cells, {n} DOF, {len(dofs)} fixed")
-    os.environ["FH_TIME_SPMV"] = "1"
+    os.environ.pop("FH_TIME_SPMV", None)
     for w in range(args.warmup):
         st = step(fh.MIXED, u_mix)
         log(f"warmup {w}: {st.iterations} it, {st.wall_ms:.1f} ms, true rel res "
             f"{st.true_rel_residual:.2e}")
 
-    # ---------------- timed region: mixed-precision steps
+    # ---------------- timed region: mixed-precision steps (HIP graphs, no event timing)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = [step(fh.MIXED, u_mix) for _ in range(args.steps)]
@@ -97,59 +97,92 @@ def main():
     total_iters = sum(int(s.iterations) for s in stats)
     value = n * total_iters / (t1 - t0)
     ms_step = (t1 - t0) * 1e3 / args.steps
-    spmv_ms = float(np.mean([s.spmv_avg_ms for s in stats]))
-    spmv_bytes = stats[-1].spmv_bytes
     for i, s in enumerate(stats):
-        log(f"step {i}: {s.iterations} it ({s.replacements} FP64 replacements), solve "
-            f"{s.kernel_ms:.1f} ms, SpMV {s.spmv_avg_ms * 1e3:.1f} us, true rel res "
-            f"{s.true_rel_residual:.2e}")
+        log(f"step {i}: {s.iterations} it ({s.iterations_low} on FP32 values, "
+            f"{s.replacements} FP64 replacements), solve {s.kernel_ms:.1f} ms, "
+            f"true rel res {s.true_rel_residual:.2e}")
 
-    # ---------------- FP64 GPU CG at equal accuracy (reference arithmetic)
-    s64 = step(fh.FP64, u_64)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    s64 = step(fh.FP64, u_64)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    fp64_value = n * int(s64.iterations) / (t3 - t2)
+    # ---------------- FP64 GPU CG at equal accuracy: FP64 difference-form SpMV,
+    # stop on the true residual (the reference's recursive-residual FP64 CG cannot
+    # certify 1e-10 on this system; it is reported beside it)
+    def timed(prec, u):
+        step(prec, u)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        st = step(prec, u)
+        torch.cuda.synchronize()
+        return st, (time.perf_counter() - ta) * 1e3
+
+    s64, ms64 = timed(fh.FP64_TRUE, u_64)
     rel_err = float(torch.linalg.norm(u_mix - u_64) / torch.linalg.norm(u_64))
-    log(f"fp64: {s64.iterations} it, {(t3 - t2) * 1e3:.1f} ms/step, SpMV "
-        f"{s64.spmv_avg_ms * 1e3:.1f} us, true rel res {s64.true_rel_residual:.2e}; "
-        f"mixed vs fp64 rel err {rel_err:.2e}")
+    log(f"fp64 (true-residual stop): {s64.iterations} it, {ms64:.1f} ms/step, true rel res "
+        f"{s64.true_rel_residual:.2e}; mixed vs fp64 rel diff {rel_err:.2e}")
+    u_ref = torch.empty_like(u_64)
+    sref, msref = timed(fh.FP64, u_ref)
+    log(f"fp64 reference semantics (recursive stop): {sref.iterations} it, {msref:.1f} ms/step, "
+        f"recursive rel res {sref.rel_residual:.2e}, true rel res {sref.true_rel_residual:.2e}")
 
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
+    # ---------------- SpMV kernel durations: one more solve of each kind with a HIP
+    # event pair around every SpMV launch on the library's stream
+    os.environ["FH_TIME_SPMV"] = "1"
+    sm = step(fh.MIXED, u_mix)
+    st64 = step(fh.FP64_TRUE, u_64)
+    os.environ.pop("FH_TIME_SPMV", None)
+    log(f"SpMV FP64 diff: {st64.spmv_avg_ms * 1e3:.1f} us over {st64.spmv_timed} launches "
+        f"({st64.spmv_bytes / 1e6:.1f} MB); mixed run: FP64 {sm.spmv_avg_ms * 1e3:.1f} us x "
+        f"{sm.spmv_timed}, FP32 {sm.spmv32_avg_ms * 1e3:.1f} us x {sm.spmv32_timed} "
+        f"({sm.spmv32_bytes / 1e6:.1f} MB)")
+
+    def gbps(nbytes, ms):
+        return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+    # dominant kernel of the mixed solve by time: FP64-value or FP32-value SpMV
+    t_hi = sm.spmv_avg_ms * sm.spmv_timed
+    t_lo = sm.spmv32_avg_ms * sm.spmv32_timed
+    if t_hi >= t_lo:
+        kname, kbytes, kms = ("k_cg_spmv<3,false,true,false> (FP64 values, difference form, "
+                              "fused p-update and p.Ap)"), sm.spmv_bytes, sm.spmv_avg_ms
+    else:
+        kname, kbytes, kms = ("k_cg_spmv<3,false,true,true> (FP32 values, difference form, "
+                              "fused p-update and p.Ap)"), sm.spmv32_bytes, sm.spmv32_avg_ms
+    achieved = gbps(kbytes, kms)
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-        "kernel": "k_cg_spmv<3,float> (FP32 BSR3 SELL-64 SpMV fused with p-update and p.Ap)",
-        "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_us": spmv_ms * 1e3,
+        "kernel": kname, "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": kms * 1e3,
+        "fp64_spmv": {"avg_launch_us": st64.spmv_avg_ms * 1e3, "bytes": st64.spmv_bytes,
+                      "gbps": gbps(st64.spmv_bytes, st64.spmv_avg_ms)},
+        "fp32_spmv": {"avg_launch_us": sm.spmv32_avg_ms * 1e3, "bytes": sm.spmv32_bytes,
+                      "gbps": gbps(sm.spmv32_bytes, sm.spmv32_avg_ms)},
     }
     pmc = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(pmc):
         try:
-            roofline["traffic"] = json.load(open(pmc)).get("k_cg_spmv_bsr3_f32_c1_bytes")
-        except Exception:
+            roofline["traffic"] = json.load(open(pmc)).get("c1_fp64_spmv_hbm_bytes")
+        except (OSError, ValueError):
             pass
 
     out = {
         "metric": METRIC, "value": value, "unit": "DOF·iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (Krylov loop) + f64 (residual, solution, reductions)",
+        "dtype": "f64 vectors and reductions; f64 then f32 matrix values",
         "data": "synthetic (structured cantilever mesh generated on the GPU)",
-        "config": {"workload": f"C1 cantilever P1 elasticity box_beam(10,2,2,{nx},{ny},{nz}), "
-                               f"{n} DOF, mixed-precision Jacobi-CG to true rel. residual 1e-10",
+        "config": {"workload": f"cantilever P1 elasticity box_beam(10,2,2,{nx},{ny},{nz}), "
+                               f"{n} DOF, assembly + condensation + mixed-precision Jacobi-CG "
+                               f"to true rel. residual 1e-10",
                    "n_dof": n, "cells": mesh.nc, "tol": tol},
         "iterations": total_iters / args.steps,
+        "iterations_fp32_values": float(np.mean([s.iterations_low for s in stats])),
         "true_rel_residual": max(s.true_rel_residual for s in stats),
-        "rel_err_vs_fp64": rel_err,
-        "spmv_gbps": achieved,
-        "fp64": {"value": fp64_value, "ms_per_step": (t3 - t2) * 1e3,
+        "rel_diff_vs_fp64": rel_err,
+        "fp64": {"value": n * int(s64.iterations) / (ms64 * 1e-3), "ms_per_step": ms64,
                  "iterations": int(s64.iterations), "true_rel_residual": s64.true_rel_residual,
-                 "spmv_avg_us": s64.spmv_avg_ms * 1e3,
-                 "spmv_gbps": s64.spmv_bytes / (s64.spmv_avg_ms * 1e-3) / 1e9
-                 if s64.spmv_avg_ms > 0 else None},
-        "speedup_vs_fp64": (t3 - t2) * 1e3 / ms_step,
+                 "mode": "FP64 difference-form SpMV, true-residual stop"},
+        "fp64_reference_semantics": {
+            "ms_per_step": msref, "iterations": int(sref.iterations),
+            "rel_residual": sref.rel_residual, "true_rel_residual": sref.true_rel_residual},
+        "speedup_vs_fp64": ms64 / ms_step,
         "roofline": roofline,
         "cpu_baseline": None,
     }

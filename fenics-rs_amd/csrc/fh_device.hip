@@ -219,7 +219,9 @@ int fh_get_device_info(int index, fh_device_info* out) {
         DeviceCtx& d = device(index);
         std::memset(out, 0, sizeof *out);
         out->index = index;
-        std::snprintf(out->name, sizeof out->name, "%s", d.prop.name);
+        // some ROCm builds leave the marketing name empty: fall back to the ISA name
+        std::snprintf(out->name, sizeof out->name, "%s",
+                      d.prop.name[0] ? d.prop.name : d.prop.gcnArchName);
         std::snprintf(out->arch, sizeof out->arch, "%s", d.prop.gcnArchName);
         size_t fr = 0, tot = 0;
         FH_HIP(hipMemGetInfo(&fr, &tot));

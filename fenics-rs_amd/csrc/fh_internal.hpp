@@ -64,6 +64,8 @@ struct fh_matrix {
     std::shared_ptr<fh::Pattern> pat;
     fh::DBuf<double> vals;   // SELL layout, [entries][bs*bs] interleaved by lane
     fh::DBuf<float> vals32;  // FP32 copy (lazily made)
+    fh::DBuf<double> rowsum64;  // [nrows][bs*bs] block row sums (difference-form SpMV)
+    fh::DBuf<float> rowsum32;   // FP32 copy of rowsum64
     fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
     uint64_t nnz_scalar = 0; // canonical CSC nnz (export)
     bool condensed = false;
@@ -113,6 +115,7 @@ uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
 // SpMV (fh_cg.hip)
 void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStream_t s);
 void inv_diag_f64(const fh_matrix* a, double* out, hipStream_t s);
+void ensure_rowsum(fh_matrix* a, bool f32);
 
 // Solve (fh_cg.hip)
 struct SolveArgs {

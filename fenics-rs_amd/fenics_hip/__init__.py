@@ -39,7 +39,7 @@ FH_ERR_BAD_INPUT = 4
 FH_ERR_HIP = 5
 FH_ERR_UNSUPPORTED = 6
 
-FP64, MIXED, MIXED_IR = 0, 1, 2
+FP64, MIXED, MIXED_IR, FP64_TRUE = 0, 1, 2, 3
 ORDER_FAST, ORDER_ORDERED = 0, 1
 
 
@@ -56,13 +56,16 @@ class DeviceInfo(C.Structure):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [("iterations", C.c_uint64), ("replacements", C.c_uint64),
+    _fields_ = [("iterations", C.c_uint64), ("iterations_low", C.c_uint64),
+                ("replacements", C.c_uint64),
                 ("final_residual", C.c_double), ("rel_residual", C.c_double),
                 ("true_rel_residual", C.c_double), ("wall_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("gflops", C.c_double),
                 ("effective_bw_gbps", C.c_double), ("spmv_bytes", C.c_double),
                 ("iter_bytes", C.c_double), ("spmv_avg_ms", C.c_double),
-                ("spmv_timed", C.c_uint64), ("device", C.c_int), ("precision", C.c_int),
+                ("spmv_timed", C.c_uint64), ("spmv32_bytes", C.c_double),
+                ("spmv32_avg_ms", C.c_double), ("spmv32_timed", C.c_uint64),
+                ("device", C.c_int), ("precision", C.c_int),
                 ("status", C.c_int)]
 
     def as_dict(self):
@@ -84,6 +87,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise FenicsError(FH_ERR_UNSUPPORTED,
                           f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    # PyTorch wheels bundle their own HIP runtime under the same soname.  When
+    # torch is importable, load it first so that the library binds to that one
+    # runtime; two HIP runtimes in one process make the second see no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     sig = {
         "fh_last_error": (C.c_char_p, []),

@@ -37,9 +37,13 @@ extern "C" {
 #define FH_ERR_UNSUPPORTED 6
 
 /* arithmetic of a solve */
-#define FH_PREC_FP64 0     /* FP64 matrix, vectors and reductions                     */
-#define FH_PREC_MIXED 1    /* FP32 CG with FP64 residual replacement (x, r, dots FP64) */
-#define FH_PREC_MIXED_IR 2 /* restarted FP32 CG inside FP64 iterative refinement       */
+#define FH_PREC_FP64 0      /* the reference's CG: FP64 SpMV, stop on the recursive residual */
+#define FH_PREC_MIXED 1     /* FP64 vectors; FP64 then FP32 matrix values once ||r||/||b|| <
+                               1e-2 (difference-form SpMV), FP64 residual replacement, stop on
+                               the true FP64 residual                                         */
+#define FH_PREC_MIXED_IR 2  /* FP32 matrix from the start, restarted at each FP64 replacement
+                               (iterative refinement), stop on the true FP64 residual        */
+#define FH_PREC_FP64_TRUE 3 /* FP64 difference-form SpMV, stop on the true FP64 residual      */
 
 /* SpMV accumulation order */
 #define FH_ORDER_FAST 0    /* fused multiply-add, any order within a row           */
@@ -58,7 +62,8 @@ typedef struct fh_device_info {
 
 /* replaces GpuSolveResult (gpu.rs:24-34) minus the solution, which goes to x_out */
 typedef struct fh_solve_stats {
-    uint64_t iterations;          /* CG iterations (inner FP32 ones for the mixed modes) */
+    uint64_t iterations;          /* CG iterations (all precisions)                      */
+    uint64_t iterations_low;      /* of which on the FP32 matrix values                  */
     uint64_t replacements;        /* FP64 residual replacements / refinement steps       */
     double final_residual;        /* cg.rs semantics: absolute recursive ||r||           */
     double rel_residual;          /* recursive ||r|| / ||b|| (gpu.rs final_residual)     */
@@ -72,6 +77,9 @@ typedef struct fh_solve_stats {
     double spmv_avg_ms;           /* mean duration of the CG SpMV kernel (HIP events), 0 if
                                      not timed (set FH_TIME_SPMV=1 to time it)           */
     uint64_t spmv_timed;          /* SpMV launches that average covers                   */
+    double spmv32_bytes;          /* algorithmic bytes of one FP32-value SpMV            */
+    double spmv32_avg_ms;         /* mean duration of the FP32-value SpMV kernel         */
+    uint64_t spmv32_timed;
     int device;
     int precision;
     int status;

@@ -3,10 +3,14 @@ tests (cg.rs:156-169, poisson_tests.rs, elasticity_tests.rs,
 ab_benchmarks.rs, gpu.rs:1609-1769).
 
 Tolerances: FP64 CG differs from the oracle's only in the summation order of
-its dot products, so solutions agree to ~kappa * 1e-16 relative (checked at
-1e-8 for tol 1e-12 solves); every solve's true FP64 residual is checked
-against its tolerance; mixed-precision solves must reach the same true
-residual and agree with the FP64 solution to 1e-7 relative."""
+its dot products.  That changes the iteration count by a few percent on
+ill-conditioned elasticity systems (the recursive residual drifts from the
+true one differently), while solutions agree to ~1e-8 relative at tol 1e-12.
+The reference semantics stop on the recursive residual, so their true FP64
+residual may exceed tol (checked within 10x).  FP64_TRUE and the mixed modes
+stop on the true FP64 residual, so that one is checked against tol itself, and
+mixed solutions agree with FP64_TRUE's to 1e-9 relative (2.4e-13 measured on a
+CPU model of the same iteration)."""
 import json
 import os
 
@@ -51,7 +55,7 @@ def test_c0_csc_drop_in_vs_oracle():
     r = fh.solve_cg_gpu_or_cpu(cp, ri, va, bb, None, 1e-12, 5000)
     assert abs(r.iterations - it_o) <= 2
     assert np.linalg.norm(r.solution - xo) / np.linalg.norm(xo) < 1e-8
-    assert r.stats.true_rel_residual < 2e-12
+    assert r.stats.true_rel_residual < 1e-11
 
 
 def test_ordered_spmv_is_bitwise_reference_spmv():
@@ -134,7 +138,7 @@ def test_cantilever_beam_bending_vs_oracle():  # elasticity_tests.rs:52-126
     fx = O.clamp_dofs_x0(om)
     Kb, bb = O.apply_dirichlet_bc(K, b, fx, np.zeros(len(fx)))
     st, uo, it, res = O.solve_cg(Kb, bb, None, 1e-9, 2000)
-    assert abs(sol.iterations - it) <= 3
+    assert abs(sol.iterations - it) <= max(3, 0.05 * it)
     assert np.linalg.norm(sol.displacements - uo) / np.linalg.norm(uo) < 1e-6
 
 
@@ -144,15 +148,38 @@ def test_mixed_precision_reaches_fp64_accuracy(prec):
     mat = fh.ElasticMaterial(210e9, 0.3)
     bc = fh.clamp_dofs_x0(m.vertices)
     f = [0, 0, -7.85e4]
-    s64 = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 20000, compute_stresses=False)
-    smx = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 40000, precision=prec,
-                                 compute_stresses=False)
-    assert s64.stats.true_rel_residual < 1.5e-10
-    assert smx.stats.true_rel_residual < 1e-10
-    u64, umx = s64.displacements, smx.displacements
-    assert np.linalg.norm(umx - u64) / np.linalg.norm(u64) < 1e-7
-    if prec == fh.MIXED:  # residual replacement keeps the Krylov space
-        assert smx.iterations < 1.3 * s64.iterations
+    kw = dict(compute_stresses=False)
+    sref = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 20000, **kw)
+    stru = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 20000, precision=fh.FP64_TRUE, **kw)
+    smx = fh.solve_elasticity_3d(m, mat, f, bc, 1e-10, 40000, precision=prec, **kw)
+    # recursive-residual stop: the plain FP64 product's rounding floor shows
+    assert sref.stats.rel_residual < 1e-10 and sref.stats.true_rel_residual < 1e-9
+    assert stru.stats.true_rel_residual < 1e-10 and smx.stats.true_rel_residual < 1e-10
+    assert stru.stats.replacements >= 1 and smx.stats.replacements >= 1
+    ut, umx = stru.displacements, smx.displacements
+    assert np.linalg.norm(umx - ut) / np.linalg.norm(ut) < 1e-9
+    assert np.linalg.norm(sref.displacements - ut) / np.linalg.norm(ut) < 1e-7
+    if prec == fh.MIXED:  # relaxation keeps the FP64 Krylov convergence
+        assert smx.iterations < 1.1 * stru.iterations
+        assert 0 < smx.stats.iterations_low < smx.iterations
+    else:
+        assert smx.stats.iterations_low == smx.iterations
+
+
+def test_fp64_true_is_reference_cg_until_tol():
+    """FP64_TRUE is cg.rs's iteration with the product evaluated in difference
+    form (same operator, different rounding) and a true-residual stop; on a
+    well-conditioned system it stops within an iteration or two of cg.rs."""
+    Ab, bb = c0_system()
+    cp, ri, va = Ab.export()
+    st, xo, it_o, res_o = O.solve_cg(Ab, bb, None, 1e-10, 5000)
+    m = fh.unit_square(128, 128)
+    A = fh.assemble_stiffness(m)
+    bd = m.boundary_dofs()
+    Ag, bg = fh.apply_dirichlet_bc(A, bb, fh.DirichletBC(bd, np.zeros(len(bd))))
+    x, s = Ag.solve(bb, tol=1e-10, max_iter=5000, precision=fh.FP64_TRUE)
+    assert abs(int(s.iterations) - it_o) <= 2 and s.true_rel_residual < 1e-10
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
 
 
 def test_error_paths():
