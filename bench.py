@@ -32,6 +32,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profile_run(iters, mesh, E, nu, fbody, bc, u):
+    """A short, fixed amount of each SpMV kind for hardware-counter runs:
+    FP64_TRUE uses the FP64-value kernel throughout, MIXED_IR the FP32-value
+    one; both stop at `iters` iterations (not converged, by design)."""
+    import torch
+    import fenics_hip as fh
+    os.environ["FH_CG_GRAPH"] = "0"  # one dispatch per kernel launch
+    K = fh.assemble_elasticity_stiffness_3d(mesh, fh.ElasticMaterial(E, nu))
+    b = fh.assemble_elasticity_body_force_3d(mesh, fbody)
+    Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
+    for prec in (fh.FP64_TRUE, fh.MIXED_IR):
+        x, st = Kb.solve(bb, tol=1e-30, max_iter=iters, precision=prec, allow_fail=True)
+        log(f"profile: precision {prec}: {st.iterations} iterations")
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0").uniform_()
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"profile: 3 device copies of {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +63,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--nx", type=int, default=200, help="cantilever cells along x (ny=nz=nx/5)")
+    ap.add_argument("--profile-iters", type=int, default=0,
+                    help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
+                         "and of the FP32-value SpMV on the cantilever, then a 1 GiB device "
+                         "copy (hardware-counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -75,7 +102,7 @@ def main():
     def step(prec, u):
         st = fh.SolveStats()
         code = L.fh_solve_elasticity_3d(mesh.h, E, nu, None, fbody.ctypes.data, len(dofs),
-                                        dofs.ctypes.data, vals.ctypes.data, tol, 200000, prec,
+                                        dofs.ctypes.data, vals.ctypes.data, tol, 50000, prec,
                                         None, C.c_void_p(u.data_ptr()), C.byref(st))
         if code != fh.FH_OK:
             raise fh.FenicsError(code, fh.last_error())
@@ -83,6 +110,9 @@ def main():
 
     log(f"mesh: {mesh.nv} vertices, {mesh.nc} cells, {n} DOF, {len(dofs)} fixed")
     os.environ.pop("FH_TIME_SPMV", None)
+    if args.profile_iters > 0:
+        profile_run(args.profile_iters, mesh, E, nu, fbody, bc, u_64)
+        return
     for w in range(args.warmup):
         st = step(fh.MIXED, u_mix)
         log(f"warmup {w}: {st.iterations} it, {st.wall_ms:.1f} ms, true rel res "

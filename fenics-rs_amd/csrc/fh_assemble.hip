@@ -392,6 +392,8 @@ static fh_matrix* new_matrix(int dev, std::shared_ptr<Pattern> pat, int bs) {
     a->n = a->pat->nrows * bs;
     a->nnz_scalar = scalar_nnz(*a->pat, bs);
     a->vals.alloc(dev, std::max<uint64_t>(a->pat->sell_entries * bs * bs, 1));
+    // SELL padding must hold zeros: the CG product reads it without a length test
+    fill_f64(a->vals.p, a->vals.n, 0.0, device(dev).stream);
     return a;
 }
 

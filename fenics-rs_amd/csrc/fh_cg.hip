@@ -30,11 +30,20 @@
 //   FH_PREC_FP64      : the reference's arithmetic (plain SpMV, stop on the
 //                       recursive residual); FH_ORDER_ORDERED makes SpMV and
 //                       axpys bitwise equal to cg.rs, only the dots differ;
-//   FH_PREC_FP64_TRUE : FP64 difference-form SpMV; when the recursive residual
-//                       reaches tol the true residual b - A x is recomputed
-//                       (difference form, FP64) and replaces r; convergence is
-//                       declared on the true residual only;
-//   FH_PREC_MIXED     : as FP64_TRUE, but once ||r||/||b|| < sw (1e-2) the
+//   FH_PREC_FP64_TRUE : FP64 difference-form SpMV and residual replacement:
+//                       whenever the recursive residual has fallen by
+//                       delta64 (0.1) since the last replacement, and when it
+//                       reaches tol, r is recomputed as the true residual
+//                       b - A x (FP64, difference form).  Replacing often keeps
+//                       the gap between the two small against the residual, so
+//                       beta can be kept (van der Vorst & Ye 2000); a gap above
+//                       10x restarts the recurrence instead.  Convergence is
+//                       declared on the true residual only (evaluated in
+//                       double-double): below tol, or at the residual that
+//                       rounding x to FP64 alone leaves (on ill-conditioned
+//                       systems such as a 1M-dof slender cantilever that floor
+//                       is ~1e-10 relative, so no FP64 vector reaches below it);
+//   FH_PREC_MIXED     : as FP64_TRUE, but once ||r||/||b|| < sw (1e-5) the
 //                       SpMV reads the FP32 copy of the matrix (difference
 //                       form, FP64 vectors and accumulation), from the next
 //                       chunk of iterations on (a separate kernel
@@ -42,14 +51,16 @@
 //                       product is only harmless when its error is small
 //                       against the residual it acts on (inexact Krylov
 //                       theory: eta_k <~ tol / ||r_k||), so the FP32 operator
-//                       is used where it cannot slow CG down; the residual is
-//                       replaced by the FP64 true one whenever it has dropped
-//                       by delta (1e-2) since the last replacement;
+//                       is used only where it cannot slow CG down; on it the
+//                       residual is replaced every delta (0.1) drop, and a
+//                       cycle whose true residual fails to halve sends the
+//                       rest of the solve back to FP64 values;
 //   FH_PREC_MIXED_IR  : FP32 operator from the start, restarting the
 //                       recurrences at each replacement: classic FP64
 //                       iterative refinement around FP32 CG (for comparison).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -72,16 +83,21 @@ struct CgState {
     uint64_t iters;
     uint64_t it_low;    // first iteration on the FP32 operator (0: none)
     uint32_t want_low;  // relaxation threshold crossed: switch to the FP32 operator
-    uint32_t pad_;
+    uint32_t stalled;   // FP32-operator cycle failed to reduce the true residual
     double rz[2];
     uint32_t low[2];    // operator precision of iteration j, by parity
     double bnorm;
     double res;
-    double r_rep;
+    double r_rep;       // residual at the last replacement (or operator switch)
+    double r_trig;      // recursive residual that triggered the pending replacement
+    uint64_t n_restart;
     double rel;
+    double rel_last;    // relative recursive residual seen by the last spmv
+    double floor_rel;   // FP64 rounding floor of the residual at the last true residual
     // configuration
     double tol;
-    double delta;
+    double delta;       // replacement factor on the FP32 operator
+    double delta64;     // replacement factor on the FP64 operator (true-residual modes)
     double sw;
     uint64_t max_iter;
     uint32_t mode;
@@ -178,43 +194,105 @@ __global__ void k_rowsum(CgMat A, double* __restrict__ S) {
     S[t] = acc;
 }
 
-// t = b - A x in FP64 (difference form when DIFF); r = t, z = D^{-1} r;
-// partials (t.t, r.z) and optionally b.b.  Initial residual and replacement.
-template <int BS, bool ORD, bool DIFF, int PRE>
+// Error-free transformations (Knuth TwoSum, FMA TwoProduct) for residuals
+// evaluated in double-double: the sum of the stored products is carried as
+// hi + lo, so b - A x is accurate to ~eps |b - A x| + eps^2 sum|a_ij||x_j|.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double a, double b) {
+    const double p = a * b;
+    const double ep = fma(a, b, -p);
+    double sum, e;
+    two_sum(hi, p, sum, e);
+    hi = sum;
+    lo += e + ep;
+}
+
+// t = b - A x for the lane's block row, in double-double (plain form, all
+// stored entries in column order).
+// `sq` accumulates sum_j (a_ij x_j)^2 over the row: eps^2 times it is the
+// expected square of (A dx)_i when x is rounded to FP64 (dx_j ~ eps x_j), the
+// residual floor no FP64 solution vector can go below.
+template <int BS>
+__device__ __forceinline__ void row_residual_dd(const CgMat& A, uint32_t s, uint32_t lane,
+                                                uint32_t len, const double* __restrict__ x,
+                                                const double* bi, double* t, double& sq) {
+    const uint32_t base = A.soff[s];
+    double hi[BS], lo[BS];
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        hi[a] = 0.0;
+        lo[a] = 0.0;
+    }
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint64_t e = (uint64_t)base + (uint64_t)k * SLICE;
+        const uint32_t c = A.cols[e + lane];
+        const double* v = A.v64 + e * BS * BS + lane;
+#pragma unroll
+        for (int q = 0; q < BS; ++q) {
+            const double xv = x[(uint64_t)c * BS + q];
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                const double av = v[(a * BS + q) * SLICE];
+                dd_add_prod(hi[a], lo[a], av, xv);
+                sq += (av * xv) * (av * xv);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        double d, e;
+        two_sum(bi[a], -hi[a], d, e);
+        t[a] = d + (e - lo[a]);
+    }
+}
+
+// t = b - A x; r = t, z = D^{-1} r; partials (t.t, r.z) and optionally b.b.
+// EXACT: t in double-double (true-residual modes: initial residual and
+// replacement); otherwise FP64 in the reference's product order (ORD).
+template <int BS, bool ORD, bool EXACT, int PRE>
 __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restrict__ b,
                                                   const double* __restrict__ x,
                                                   const double* __restrict__ dinv,
                                                   double* __restrict__ r, double* __restrict__ z,
                                                   double2* __restrict__ red,
-                                                  double* __restrict__ bb) {
+                                                  double* __restrict__ bb,
+                                                  double* __restrict__ fl) {
     __shared__ double sh[4];
     const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
     const unsigned s = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    double tt = 0.0, rz = 0.0, bsq = 0.0;
+    double tt = 0.0, rz = 0.0, bsq = 0.0, sq = 0.0;
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
         const bool live = row < A.nrows;
         const uint32_t len = live ? A.rlen[row] : 0;
-        double xr[BS], acc[BS];
+        double rv[BS];
+        if constexpr (EXACT) {
+            double bi[BS];
 #pragma unroll
-        for (int q = 0; q < BS; ++q) xr[q] = live ? x[(uint64_t)row * BS + q] : 0.0;
-        row_product<BS, double, ORD, DIFF>(
-            A, A.v64, s, lane, len, xr,
-            [&](uint32_t c, double* xv) {
+            for (int a = 0; a < BS; ++a) bi[a] = live ? b[(uint64_t)row * BS + a] : 0.0;
+            row_residual_dd<BS>(A, s, lane, len, x, bi, rv, sq);
+        } else {
+            double xr[BS], acc[BS];
+            row_product<BS, double, ORD, false>(
+                A, A.v64, s, lane, len, xr,
+                [&](uint32_t c, double* xv) {
 #pragma unroll
-                for (int q = 0; q < BS; ++q) xv[q] = x[(uint64_t)c * BS + q];
-            },
-            acc);
+                    for (int q = 0; q < BS; ++q) xv[q] = x[(uint64_t)c * BS + q];
+                },
+                acc);
+#pragma unroll
+            for (int a = 0; a < BS; ++a) rv[a] = live ? b[(uint64_t)row * BS + a] - acc[a] : 0.0;
+        }
         if (live) {
-            if (DIFF) add_rowsum<BS>(A.S64, row, xr, acc);
-            double rv[BS];
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
                 const uint64_t i = (uint64_t)row * BS + a;
-                const double t = b[i] - acc[a];
-                tt += t * t;
-                rv[a] = t;
-                r[i] = t;
+                tt += rv[a] * rv[a];
+                r[i] = rv[a];
                 if (bb) bsq += b[i] * b[i];
             }
             if constexpr (PRE == 0) {
@@ -239,16 +317,24 @@ __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restr
     tt = fhd::block_sum256(tt, sh);
     rz = fhd::block_sum256(rz, sh);
     if (bb) bsq = fhd::block_sum256(bsq, sh);
+    if (EXACT) sq = fhd::block_sum256(sq, sh);
     if (threadIdx.x == 0) {
         red[blockIdx.x] = make_double2(tt, rz);
         if (bb) bb[blockIdx.x] = bsq;
+        if (EXACT) fl[blockIdx.x] = sq;
     }
 }
 
 // ---------------------------------------------------------------- CG kernels
 __device__ __forceinline__ double2 reduce_pairs(const double2* __restrict__ p, int m, double* sh) {
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < m; i += 256) {
+    int i = threadIdx.x;
+    for (; i + 768 < m; i += 1024) {  // four independent loads in flight
+        const double2 u = p[i], v = p[i + 256], w = p[i + 512], t = p[i + 768];
+        a += (u.x + v.x) + (w.x + t.x);
+        b += (u.y + v.y) + (w.y + t.y);
+    }
+    for (; i < m; i += 256) {
         const double2 v = p[i];
         a += v.x;
         b += v.y;
@@ -258,14 +344,77 @@ __device__ __forceinline__ double2 reduce_pairs(const double2* __restrict__ p, i
     return make_double2(a, b);
 }
 
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// Software-pipelined form of row_product for the CG loop (fused-multiply-add
+// order): U stored blocks per step, the column indices of the next step
+// loaded before this step's gathers, matrix values and indices streamed with
+// non-temporal loads so that the gathered vectors keep the cache.  No per-lane
+// length test: padded SELL entries hold zero values and the row's own column
+// (or column 0 past the last row), so they add exact zeros.
+template <int BS, int U, class TM, bool DIFF, class G>
+__device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __restrict__ vals,
+                                                 uint32_t s, uint32_t lane, const double* xr,
+                                                 G&& gather, double* acc) {
+    constexpr int B2 = BS * BS;
+    const uint32_t base = A.soff[s], w = (A.soff[s + 1] - base) / SLICE;
+#pragma unroll
+    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
+    if (w == 0) return;
+    const uint32_t* cp = A.cols + (uint64_t)base + lane;
+    const TM* vp = vals + (uint64_t)base * B2 + lane;
+    const uint32_t last = w - 1;
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = ld_nt(cp + (uint64_t)min((uint32_t)u, last) * SLICE);
+    for (uint32_t k = 0; k < w; k += U) {
+        uint32_t cn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            cn[u] = ld_nt(cp + (uint64_t)min(k + U + u, last) * SLICE);
+        TM v[U][B2];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const TM* vk = vp + (uint64_t)min(k + u, last) * SLICE * B2;
+#pragma unroll
+            for (int j = 0; j < B2; ++j) v[u][j] = ld_nt(vk + j * SLICE);
+        }
+        double xg[U][BS];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            gather(c[u], xg[u]);
+            // steps past the slice width re-read its last entry: add zero
+            const bool ok = u == 0 || k + u < w;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) {
+                const double d = DIFF ? xg[u][q] - xr[q] : xg[u][q];
+                xg[u][q] = ok ? d : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int a = 0; a < BS; ++a)
+#pragma unroll
+                for (int q = 0; q < BS; ++q)
+                    acc[a] = fma((double)v[u][a * BS + q], xg[u][q], acc[a]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = cn[u];
+    }
+}
+
 // LOW: the product reads the FP32 copy of the matrix (difference form).  The
 // host picks the instantiation per chunk of iterations; the FP64 one raises
 // st->want_low when the relaxation threshold is crossed.
 template <int BS, bool ORD, bool DIFF, bool LOW>
-__global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ z, double* p0,
+__global__ __launch_bounds__(256, (BS == 3 && !DIFF) ? 5 : 6) void k_cg_spmv(CgMat A, const double* __restrict__ z, double* p0,
                                                  double* p1, double* __restrict__ Ap, CgState* st,
                                                  const double2* __restrict__ red_in, int n_in,
                                                  const double* __restrict__ bb_in,
+                                                 const double* __restrict__ fl_in,
                                                  double* __restrict__ red_out) {
     static_assert(DIFF || !LOW, "the FP32 operator is only used in difference form");
     __shared__ double sh[4];
@@ -284,11 +433,16 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
         const double bnorm = sqrt(fhd::reduce_partials256(bb_in, n_in, sh));
         const double r0 = sqrt(rr);
         rel = bnorm > 0.0 ? r0 / bnorm : r0;
+        // true-residual modes: also stop at the FP64 rounding floor of x
+        const double floor0 =
+            mode != M_REF ? 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh))
+                          : 0.0;
         if (lead) {
             st->bnorm = bnorm;
             st->r_rep = r0;
+            st->floor_rel = bnorm > 0.0 ? floor0 / bnorm : floor0;
         }
-        if (r0 < tol || (bnorm > 0.0 && rel < tol)) {
+        if (r0 < tol || (bnorm > 0.0 && rel < tol) || (mode != M_REF && r0 <= floor0)) {
             if (lead) {
                 st->status = ST_CONVERGED;
                 st->iters = 0;
@@ -303,8 +457,14 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
         const double res = sqrt(rr);
         rel = bnorm > 0.0 ? res / bnorm : res;  // cg.rs:118-119
         const bool fresh = mode != M_REF && st->rep_mark == j;
+        // a fresh (true) residual at the FP64 rounding floor of x is as small
+        // as any FP64 solution's: converged to attainable accuracy
+        const double floorr =
+            fresh ? 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh))
+                  : 0.0;
+        if (lead && fresh) st->floor_rel = bnorm > 0.0 ? floorr / bnorm : floorr;
         if (mode == M_REF || fresh) {
-            if (rel < tol) {  // cg.rs:120-122
+            if (rel < tol || (fresh && res <= floorr)) {  // cg.rs:120-122
                 if (lead) {
                     st->status = ST_CONVERGED;
                     st->iters = j - 1;
@@ -313,8 +473,11 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
                 }
                 return;
             }
-        } else if (rel < tol || (low_prev && res < st->delta * st->r_rep)) {
-            if (lead) st->need_rep = 1;  // recompute r as the FP64 true residual first
+        } else if (rel < tol || res < (low_prev ? st->delta : st->delta64) * st->r_rep) {
+            if (lead) {
+                st->need_rep = 1;  // recompute r as the FP64 true residual first
+                st->r_trig = res;
+            }
             return;
         }
         if (j > st->max_iter) {
@@ -326,8 +489,20 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
             }
             return;
         }
-        if (lead && (fresh || (LOW && !low_prev))) st->r_rep = res;
-        beta = (fresh && st->restart) ? 0.0 : rz / st->rz[(j - 1) & 1];
+        if (lead) st->rel_last = rel;
+        if (lead && (fresh || (LOW && !low_prev))) {
+            // a replacement cycle on the FP32 operator in which the recursive
+            // residual fell by delta but the true one not even by 2: the FP32
+            // operator's error has reached the residual; finish on FP64 values
+            if (fresh && low_prev && res > 0.5 * st->r_rep) st->stalled = 1;
+            st->r_rep = res;
+        }
+        // keep the recurrence across a replacement (van der Vorst & Ye) unless
+        // the true residual is far from the recursive one it replaces: then
+        // the Krylov information is stale and CG restarts from the true one
+        const bool restart = fresh && (st->restart || res > 10.0 * st->r_trig);
+        if (lead && restart) st->n_restart = st->n_restart + 1;
+        beta = restart ? 0.0 : rz / st->rz[(j - 1) & 1];
     }
     if (lead) {
         st->rz[j & 1] = rz;
@@ -340,7 +515,9 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
     double* pnew = (j & 1) ? p1 : p0;
 
     const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
-    const unsigned s = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
+    const unsigned s = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+    const unsigned lane = threadIdx.x & 63;
     double dot = 0.0;
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
@@ -359,11 +536,14 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
                 xv[q] = madd(z[ci], beta, pold[ci], ORD);
             }
         };
+        constexpr int U = BS == 1 ? 4 : 2;
         if constexpr (LOW) {
-            row_product<BS, float, false, true>(A, A.v32, s, lane, len, xr, gather, acc);
+            row_product_pipe<BS, U, float, true>(A, A.v32, s, lane, xr, gather, acc);
             if (live) add_rowsum<BS>(A.S32, row, xr, acc);
+        } else if constexpr (ORD) {
+            row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
         } else {
-            row_product<BS, double, ORD, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
+            row_product_pipe<BS, U, double, DIFF>(A, A.v64, s, lane, xr, gather, acc);
             if (DIFF && live) add_rowsum<BS>(A.S64, row, xr, acc);
         }
         if (live) {
@@ -480,7 +660,7 @@ __global__ void k_convert(const TI* __restrict__ a, TO* __restrict__ b, uint64_t
         b[i] = (TO)a[i];
 }
 
-// ||b - A x||^2 (difference form, FP64) and ||b||^2 block partials
+// ||b - A x||^2 (double-double evaluation) and ||b||^2 block partials
 template <int BS>
 __global__ __launch_bounds__(256) void k_true_res(CgMat A, const double* __restrict__ b,
                                                   const double* __restrict__ x,
@@ -491,26 +671,16 @@ __global__ __launch_bounds__(256) void k_true_res(CgMat A, const double* __restr
     double tt = 0.0, bsq = 0.0;
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
-        const bool live = row < A.nrows;
-        const uint32_t len = live ? A.rlen[row] : 0;
-        double xr[BS], acc[BS];
+        if (row < A.nrows) {
+            double bi[BS], t[BS];
 #pragma unroll
-        for (int q = 0; q < BS; ++q) xr[q] = live ? x[(uint64_t)row * BS + q] : 0.0;
-        row_product<BS, double, false, true>(
-            A, A.v64, s, lane, len, xr,
-            [&](uint32_t c, double* xv) {
-#pragma unroll
-                for (int q = 0; q < BS; ++q) xv[q] = x[(uint64_t)c * BS + q];
-            },
-            acc);
-        if (live) {
-            add_rowsum<BS>(A.S64, row, xr, acc);
+            for (int a = 0; a < BS; ++a) bi[a] = b[(uint64_t)row * BS + a];
+            double sq = 0.0;
+            row_residual_dd<BS>(A, s, lane, A.rlen[row], x, bi, t, sq);
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
-                const uint64_t i = (uint64_t)row * BS + a;
-                const double t = b[i] - acc[a];
-                tt += t * t;
-                bsq += b[i] * b[i];
+                tt += t[a] * t[a];
+                bsq += bi[a] * bi[a];
             }
         }
     }
@@ -636,15 +806,16 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     double* x = args.x;
     if (args.x0) d2d(x, args.x0, n, s);
     else fill_f64(x, n, 0.0, s);
-    DBuf<double> redA(dev, gs), bb(dev, gs);
+    DBuf<double> redA(dev, gs), bb(dev, gs), fl(dev, gs);
     DBuf<double2> redB(dev, std::max(gs, gu));
 
     CgState* hs = host_state(dev);
     std::memset(hs, 0, sizeof *hs);
     hs->tol = args.tol;
     hs->max_iter = args.max_iter;
-    hs->delta = env_dbl("FH_RR_DELTA", mode == M_MIXED_IR ? 1e-4 : 1e-2);
-    hs->sw = env_dbl("FH_MIXED_SWITCH", 1e-2);
+    hs->delta = env_dbl("FH_RR_DELTA", 1e-1);
+    hs->delta64 = env_dbl("FH_RR_DELTA64", 1e-1);
+    hs->sw = env_dbl("FH_MIXED_SWITCH", 1e-5);
     hs->mode = mode;
     hs->restart = mode == M_MIXED_IR ? 1 : 0;
     hs->rep_mark = 1;
@@ -654,7 +825,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     Timer tk(s);
     // r0 = b - A x0 (FP64), z0 = D^{-1} r0, partials (r.r, r.z), b.b
     hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, args.b, x,
-                       dinv.p, r.p, z.p, redB.p, bb.p);
+                       dinv.p, r.p, z.p, redB.p, bb.p, fl.p);
     FH_LAUNCH_CHECK("residual");
     int n_in_spmv = (int)gs;  // partial count feeding the next spmv kernel
 
@@ -662,6 +833,10 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     const bool use_graph = env_int("FH_CG_GRAPH", 1) != 0;
     // optional per-launch timing of the SpMV kernel with HIP events on this stream
     const bool time_spmv = env_int("FH_TIME_SPMV", 0) != 0;
+    const bool trace = env_int("FH_CG_TRACE", 0) != 0;  // per-chunk state on stderr
+    // FP64-value phase of the true-residual modes: difference-form product
+    // (smaller rounding error between replacements) or the plain one (fewer bytes)
+    const bool diff64 = DIFF && env_int("FH_CG_DIFF64", 1) != 0;
     std::vector<hipEvent_t> ev;
     double ms_hi = 0.0, ms_lo = 0.0;
     uint64_t n_hi = 0, n_lo = 0;
@@ -675,14 +850,17 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             const int nin = m == 0 ? nin_first : (int)gu;
             if constexpr (DIFF) {
                 if (lowk)
-                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, DIFF, true>), dim3(gs), dim3(256), 0, s,
-                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, redA.p);
+                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, true, true>), dim3(gs), dim3(256), 0, s,
+                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
+                else if (diff64)
+                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, true, false>), dim3(gs), dim3(256), 0, s,
+                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
                 else
-                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, DIFF, false>), dim3(gs), dim3(256), 0, s,
-                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, redA.p);
+                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, false, false>), dim3(gs), dim3(256), 0, s,
+                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
             } else {
                 hipLaunchKernelGGL((k_cg_spmv<BS, ORD, false, false>), dim3(gs), dim3(256), 0, s, A,
-                                   z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, redA.p);
+                                   z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
             }
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m + 1], s));
             hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, n, x, r.p, Ap.p,
@@ -731,11 +909,18 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             }
         }
         if (hs->status != ST_RUNNING) break;
-        if (hs->want_low) lowk = true;
+        if (hs->want_low && !hs->stalled) lowk = true;
+        if (hs->stalled) lowk = false;
+        if (trace)
+            std::fprintf(stderr,
+                         "cg: it %llu rel %.3e rep %llu restarts %llu low %d need_rep %d stalled %d\n",
+                         (unsigned long long)hs->it_upd, hs->rel_last, (unsigned long long)hs->n_rep,
+                         (unsigned long long)hs->n_restart, (int)lowk, (int)hs->need_rep,
+                         (int)hs->stalled);
         if (hs->need_rep) {
-            // residual replacement: r = b - A64 x (difference form, FP64)
+            // residual replacement: r = b - A x (double-double evaluation)
             hipLaunchKernelGGL((k_residual<BS, false, true, PRE>), dim3(gs), dim3(256), 0, s, A,
-                               args.b, x, dinv.p, r.p, z.p, redB.p, (double*)nullptr);
+                               args.b, x, dinv.p, r.p, z.p, redB.p, (double*)nullptr, fl.p);
             hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, st.p);
             FH_LAUNCH_CHECK("replacement");
             n_in_spmv = (int)gs;
@@ -776,9 +961,10 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         stats->final_residual = hs->res;
         stats->rel_residual = hs->rel;
         stats->true_rel_residual = bsq > 0 ? std::sqrt(tt) / std::sqrt(bsq) : std::sqrt(tt);
+        stats->residual_floor = hs->floor_rel;
         stats->wall_ms = wall;
         stats->kernel_ms = kms;
-        stats->spmv_bytes = DIFF ? diff_spmv_bytes(a, 8) : spmv_algorithmic_bytes(a, 8, 8);
+        stats->spmv_bytes = diff64 ? diff_spmv_bytes(a, 8) : spmv_algorithmic_bytes(a, 8, 8);
         stats->spmv32_bytes = diff_spmv_bytes(a, 4);
         // SpMV + the vector passes of the fused loop: the spmv kernel gathers
         // z and p_old and writes p_new and Ap, the update reads x, r, Ap, p,

@@ -192,7 +192,12 @@ __device__ __forceinline__ double block_sum256(double v, double* sh) {
 __device__ __forceinline__ double reduce_partials256(const double* __restrict__ part, int m,
                                                      double* sh) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += 256) s += part[i];
+    int i = threadIdx.x;
+    for (; i + 768 < m; i += 1024) {  // four independent loads in flight
+        const double a = part[i], b = part[i + 256], c = part[i + 512], d = part[i + 768];
+        s += (a + b) + (c + d);
+    }
+    for (; i < m; i += 256) s += part[i];
     return block_sum256(s, sh);
 }
 

@@ -190,7 +190,10 @@ void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s) {
                        dS.p, Ke.p);
     FH_LAUNCH_CHECK("p2_elem_mfma");
     const uint64_t nv = a->pat->sell_entries;
-    if (!a->vals32.p) a->vals32.alloc(m->dev, std::max<uint64_t>(nv, 1));
+    if (!a->vals32.p) {
+        a->vals32.alloc(m->dev, std::max<uint64_t>(nv, 1));
+        fill_f32(a->vals32.p, a->vals32.n, 0.0f, s);  // zero SELL padding
+    }
     hipLaunchKernelGGL(k_gather_p2_f32, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s,
                        p2.adj->ptr.p, p2.adj->ent.p, p2.adj->slots.p, p2.pat->slice_off.p,
                        p2.pat->row_len.p, p2.adj->ndofs, Ke.p, a->vals32.p);

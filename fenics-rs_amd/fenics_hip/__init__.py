@@ -59,7 +59,8 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("iterations_low", C.c_uint64),
                 ("replacements", C.c_uint64),
                 ("final_residual", C.c_double), ("rel_residual", C.c_double),
-                ("true_rel_residual", C.c_double), ("wall_ms", C.c_double),
+                ("true_rel_residual", C.c_double), ("residual_floor", C.c_double),
+                ("wall_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("gflops", C.c_double),
                 ("effective_bw_gbps", C.c_double), ("spmv_bytes", C.c_double),
                 ("iter_bytes", C.c_double), ("spmv_avg_ms", C.c_double),

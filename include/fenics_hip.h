@@ -38,12 +38,13 @@ extern "C" {
 
 /* arithmetic of a solve */
 #define FH_PREC_FP64 0      /* the reference's CG: FP64 SpMV, stop on the recursive residual */
-#define FH_PREC_MIXED 1     /* FP64 vectors; FP64 then FP32 matrix values once ||r||/||b|| <
-                               1e-2 (difference-form SpMV), FP64 residual replacement, stop on
-                               the true FP64 residual                                         */
-#define FH_PREC_MIXED_IR 2  /* FP32 matrix from the start, restarted at each FP64 replacement
-                               (iterative refinement), stop on the true FP64 residual        */
-#define FH_PREC_FP64_TRUE 3 /* FP64 difference-form SpMV, stop on the true FP64 residual      */
+#define FH_PREC_MIXED 1     /* as FH_PREC_FP64_TRUE, with FP32 matrix values (difference-form
+                               SpMV, FP64 vectors) once ||r||/||b|| < 1e-5                    */
+#define FH_PREC_MIXED_IR 2  /* FP32 matrix values from the start, restarted at each FP64
+                               replacement (iterative refinement)                            */
+#define FH_PREC_FP64_TRUE 3 /* FP64 difference-form SpMV with periodic replacement by the
+                               double-double true residual; stops when the true residual is
+                               below tol or at the FP64 rounding floor (residual_floor)      */
 
 /* SpMV accumulation order */
 #define FH_ORDER_FAST 0    /* fused multiply-add, any order within a row           */
@@ -67,7 +68,11 @@ typedef struct fh_solve_stats {
     uint64_t replacements;        /* FP64 residual replacements / refinement steps       */
     double final_residual;        /* cg.rs semantics: absolute recursive ||r||           */
     double rel_residual;          /* recursive ||r|| / ||b|| (gpu.rs final_residual)     */
-    double true_rel_residual;     /* ||b - A x|| / ||b|| recomputed in FP64 on the GPU    */
+    double true_rel_residual;     /* ||b - A x|| / ||b||, A x summed in double-double     */
+    double residual_floor;        /* true-residual modes: relative residual that rounding
+                                     x to FP64 alone leaves, 2 eps ||(sum_j (a_ij x_j)^2)^.5||
+                                     / ||b||; a solve also converges when its true
+                                     residual reaches it (0 for FH_PREC_FP64)          */
     double wall_ms;
     double kernel_ms;
     double gflops;

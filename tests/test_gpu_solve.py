@@ -162,8 +162,8 @@ def test_mixed_precision_reaches_fp64_accuracy(prec):
     if prec == fh.MIXED:  # relaxation keeps the FP64 Krylov convergence
         assert smx.iterations < 1.1 * stru.iterations
         assert 0 < smx.stats.iterations_low < smx.iterations
-    else:
-        assert smx.stats.iterations_low == smx.iterations
+    else:  # FP32 values from the first iteration (FP64 again only after a stall)
+        assert smx.stats.iterations_low > 0
 
 
 def test_fp64_true_is_reference_cg_until_tol():
