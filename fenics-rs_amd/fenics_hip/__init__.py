@@ -29,7 +29,9 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfenics_hip.so")
+# FENICS_HIP_LIB selects another build of the library (e.g. an experimental one)
+LIB_PATH = os.environ.get("FENICS_HIP_LIB") or os.path.join(os.path.dirname(_HERE),
+                                                           "libfenics_hip.so")
 
 FH_OK = 0
 FH_ERR_NO_DEVICE = 1
