@@ -64,7 +64,7 @@ struct fh_matrix {
     std::shared_ptr<fh::Pattern> pat;
     fh::DBuf<double> vals;   // SELL layout, [entries][bs*bs] interleaved by lane
     fh::DBuf<float> vals32;  // FP32 copy (lazily made)
-    fh::DBuf<double> rowsum64;  // [nrows][bs*bs] block row sums (difference-form SpMV)
+    fh::DBuf<double> rowsum64;  // block row sums, lane-interleaved [slice][bs*bs][64] (difference form)
     fh::DBuf<float> rowsum32;   // FP32 copy of rowsum64
     fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
     uint64_t nnz_scalar = 0; // canonical CSC nnz (export)

@@ -42,7 +42,7 @@ extern "C" {
                                SpMV, FP64 vectors) once ||r||/||b|| < 1e-5                    */
 #define FH_PREC_MIXED_IR 2  /* FP32 matrix values from the start, restarted at each FP64
                                replacement (iterative refinement)                            */
-#define FH_PREC_FP64_TRUE 3 /* FP64 difference-form SpMV with periodic replacement by the
+#define FH_PREC_FP64_TRUE 3 /* FP64 SpMV with periodic replacement by the
                                double-double true residual; stops when the true residual is
                                below tol or at the FP64 rounding floor (residual_floor)      */
 
