@@ -7,9 +7,12 @@ linear-elasticity P1 cantilever box_beam(10, 2, 2, 200, 40, 40) (1,013,643 DOF,
 
 One step = the reference's solve_elasticity_3d hot path on device-resident
 inputs: numeric assembly of K and the body force, Dirichlet condensation,
-and Jacobi-PCG to ||b - A x|| / ||b|| < 1e-10.  The symbolic pattern is built
-once per mesh in the warm-up (as the mesh itself, it depends only on the
-connectivity).  Prints one JSON line.
+and Jacobi-PCG until the true residual ||b - A x|| / ||b|| (A x summed in
+double-double) is below 1e-10 or at the floor that rounding x to FP64 leaves
+(2-9e-10 on this system, where the reference's recursive-residual CG stops at
+a true 9.6e-9).  The symbolic pattern is built once per mesh in the warm-up
+(as the mesh itself, it depends only on the connectivity).  Prints one JSON
+line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -201,14 +204,16 @@ def main():
         "data": "synthetic (structured cantilever mesh generated on the GPU)",
         "config": {"workload": f"cantilever P1 elasticity box_beam(10,2,2,{nx},{ny},{nz}), "
                                f"{n} DOF, assembly + condensation + mixed-precision Jacobi-CG "
-                               f"to true rel. residual 1e-10",
+                               f"to true rel. residual 1e-10 or the FP64 floor",
                    "n_dof": n, "cells": mesh.nc, "tol": tol},
         "iterations": total_iters / args.steps,
         "iterations_fp32_values": float(np.mean([s.iterations_low for s in stats])),
         "true_rel_residual": max(s.true_rel_residual for s in stats),
+        "residual_floor": stats[-1].residual_floor,
         "rel_diff_vs_fp64": rel_err,
         "fp64": {"value": n * int(s64.iterations) / (ms64 * 1e-3), "ms_per_step": ms64,
                  "iterations": int(s64.iterations), "true_rel_residual": s64.true_rel_residual,
+                 "residual_floor": s64.residual_floor,
                  "mode": "FP64 SpMV, double-double residual replacement, true-residual stop"},
         "fp64_reference_semantics": {
             "ms_per_step": msref, "iterations": int(sref.iterations),

@@ -26,7 +26,7 @@ using namespace fhg;
 // value index of (block row r, entry k, block component j) in SELL-64
 __device__ __forceinline__ uint64_t vidx(uint32_t soff_s, uint32_t k, int j, uint32_t lane,
                                          int bs2) {
-    return ((uint64_t)soff_s + (uint64_t)k * SLICE) * bs2 + (uint64_t)j * SLICE + lane;
+    return vpos((uint64_t)soff_s + (uint64_t)k * SLICE, j, lane, bs2);
 }
 
 struct RowCtx {
@@ -574,8 +574,8 @@ static HostCsr export_scalar_csr(const fh_matrix* a) {
                 for (int be = 0; be < bs; ++be) {
                     const uint64_t c = (uint64_t)cb * bs + be;
                     if (a->condensed && (fixed[i] || fixed[c]) && i != c) continue;
-                    const uint64_t ix = ((uint64_t)soff[s] + (uint64_t)k * SLICE) * bs2 +
-                                        (uint64_t)(al * bs + be) * SLICE + l;
+                    const uint64_t ix =
+                        vpos((uint64_t)soff[s] + (uint64_t)k * SLICE, al * bs + be, l, bs2);
                     h.ci.push_back((uint32_t)c);
                     h.v.push_back(vals[ix]);
                 }

@@ -39,7 +39,7 @@
 //                       whenever the recursive residual has fallen by
 //                       delta64 (0.1) since the last replacement, and when it
 //                       reaches tol, r is recomputed as the true residual
-//                       b - A x (FP64, difference form).  Replacing often keeps
+//                       b - A x (A x summed in double-double).  Replacing often keeps
 //                       the gap between the two small against the residual, so
 //                       beta can be kept (van der Vorst & Ye 2000); a gap above
 //                       10x restarts the recurrence instead.  Convergence is
@@ -48,7 +48,7 @@
 //                       rounding x to FP64 alone leaves (on ill-conditioned
 //                       systems such as a 1M-dof slender cantilever that floor
 //                       is ~1e-10 relative, so no FP64 vector reaches below it);
-//   FH_PREC_MIXED     : as FP64_TRUE, but once ||r||/||b|| < sw (1e-5) the
+//   FH_PREC_MIXED     : as FP64_TRUE, but once ||r||/||b|| < sw (1e-3) the
 //                       SpMV reads the FP32 copy of the matrix (difference
 //                       form, FP64 vectors and accumulation), from the next
 //                       chunk of iterations on (a separate kernel
@@ -58,8 +58,9 @@
 //                       theory: eta_k <~ tol / ||r_k||), so the FP32 operator
 //                       is used only where it cannot slow CG down; on it the
 //                       residual is replaced every delta (0.1) drop, and a
-//                       cycle whose true residual fails to halve sends the
-//                       rest of the solve back to FP64 values;
+//                       cycle whose true residual fails to halve, or that
+//                       takes over 1.25x the iterations of the last FP64
+//                       cycle, sends the rest of the solve back to FP64 values;
 //   FH_PREC_MIXED_IR  : FP32 operator from the start, restarting the
 //                       recurrences at each replacement: classic FP64
 //                       iterative refinement around FP32 CG (for comparison).
@@ -93,12 +94,22 @@ struct CgState {
     uint32_t low[2];    // operator precision of iteration j, by parity
     double bnorm;
     double res;
-    double r_rep;       // residual at the last replacement (or operator switch)
+    // Fields written by block 0 of a kernel are never what other blocks of the
+    // same kernel decide on: per-replacement values are double-buffered by the
+    // parity of n_rep (advanced between kernels, by k_mark), per-iteration
+    // values by the parity of the iteration.
+    double r_rep[2];    // true residual at replacement n (or the recursive one
+                        // where the FP32 operator took over), by n_rep parity
+    uint32_t n_stag[2]; // consecutive replacement cycles that did not halve it
+    uint32_t pad2_;
     double r_trig;      // recursive residual that triggered the pending replacement
     uint64_t n_restart;
     double rel;
     double rel_last;    // relative recursive residual seen by the last spmv
     double floor_rel;   // FP64 rounding floor of the residual at the last true residual
+    uint64_t rep_it;    // iterations done at the last replacement
+    uint64_t cyc_last;  // length of the last replacement cycle (0: none yet)
+    uint64_t cyc_hi;    // length of the last replacement cycle on FP64 values (0: none)
     // configuration
     double tol;
     double delta;       // replacement factor on the FP32 operator
@@ -150,12 +161,11 @@ __device__ __forceinline__ void row_product(const CgMat& A, const TM* __restrict
             if (DIFF)
 #pragma unroll
                 for (int q = 0; q < BS; ++q) xv[q] = xv[q] - xr[q];
-            const TM* v = vals + e * BS * BS + lane;
 #pragma unroll
             for (int a = 0; a < BS; ++a)
 #pragma unroll
                 for (int q = 0; q < BS; ++q)
-                    acc[a] = madd(acc[a], (double)v[(a * BS + q) * SLICE], xv[q], ORD);
+                    acc[a] = madd(acc[a], (double)vals[vpos(e, a * BS + q, lane, BS * BS)], xv[q], ORD);
         }
     }
 }
@@ -207,7 +217,7 @@ __global__ void k_rowsum(CgMat A, double* __restrict__ S) {
     const uint32_t len = row < A.nrows ? A.rlen[row] : 0, base = A.soff[s];
     double acc = 0.0;
     for (uint32_t k = 0; k < len; ++k)
-        acc += A.v64[((uint64_t)base + (uint64_t)k * SLICE) * B2 + (uint64_t)j * SLICE + lane];
+        acc += A.v64[vpos((uint64_t)base + (uint64_t)k * SLICE, j, lane, B2)];
     S[t] = acc;
 }
 
@@ -247,13 +257,12 @@ __device__ __forceinline__ void row_residual_dd(const CgMat& A, uint32_t s, uint
     for (uint32_t k = 0; k < len; ++k) {
         const uint64_t e = (uint64_t)base + (uint64_t)k * SLICE;
         const uint32_t c = A.cols[e + lane];
-        const double* v = A.v64 + e * BS * BS + lane;
 #pragma unroll
         for (int q = 0; q < BS; ++q) {
             const double xv = x[li<BS>(c, q)];
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
-                const double av = v[(a * BS + q) * SLICE];
+                const double av = A.v64[vpos(e, a * BS + q, lane, BS * BS)];
                 dd_add_prod(hi[a], lo[a], av, xv);
                 sq += (av * xv) * (av * xv);
             }
@@ -378,6 +387,15 @@ __device__ __forceinline__ T ld_s(const T* p) {
     else return *p;
 }
 
+// two consecutive values (16 bytes for FP64, 8 for FP32) in one load
+template <bool NT, class T>
+__device__ __forceinline__ void ld_pair(const T* p, T& a, T& b) {
+    typedef T pair_t __attribute__((ext_vector_type(2)));
+    const pair_t v = ld_s<NT>(reinterpret_cast<const pair_t*>(p));
+    a = v.x;
+    b = v.y;
+}
+
 // Software-pipelined form of row_product for the CG loop (fused-multiply-add
 // order): U stored blocks per step, the column indices of the next step
 // loaded before this step's gathers.  No per-lane length test: padded SELL
@@ -393,7 +411,7 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
     if (w == 0) return;
     const uint32_t* cp = A.cols + (uint64_t)base + lane;
-    const TM* vp = vals + (uint64_t)base * B2 + lane;
+    const TM* vb = vals + (uint64_t)base * B2;  // step k: vb + k * SLICE * B2, see vpos
     const uint32_t last = w - 1;
     uint32_t c[U];
 #pragma unroll
@@ -406,9 +424,16 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
         TM v[U][B2];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const TM* vk = vp + (uint64_t)min(k + u, last) * SLICE * B2;
+            const TM* vk = vb + (uint64_t)min(k + u, last) * SLICE * B2;
+            if constexpr (B2 == 9) {
 #pragma unroll
-            for (int j = 0; j < B2; ++j) v[u][j] = ld_s<NT>(vk + j * SLICE);
+                for (int pp = 0; pp < 4; ++pp)
+                    ld_pair<NT>(vk + 128 * pp + 2 * lane, v[u][2 * pp], v[u][2 * pp + 1]);
+                v[u][8] = ld_s<NT>(vk + 512 + lane);
+            } else {
+#pragma unroll
+                for (int j = 0; j < B2; ++j) v[u][j] = ld_s<NT>(vk + j * SLICE + lane);
+            }
         }
         double xg[U][BS];
 #pragma unroll
@@ -437,7 +462,11 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 // LOW: the product reads the FP32 copy of the matrix (difference form).  The
 // host picks the instantiation per chunk of iterations; the FP64 one raises
 // st->want_low when the relaxation threshold is crossed.
-template <int BS, bool ORD, bool DIFF, bool LOW>
+// REC: A p_new = A z + beta A p_old (Chronopoulos & Gear 1989): only z is
+// gathered, A p_old is the previous Ap of the lane's own row.  The recurrence
+// drifts from the exact product by rounding, so it is used only in the
+// true-residual modes, whose replacements follow an exact product.
+template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false>
 __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ z, double* p0,
                                                  double* p1, double* __restrict__ Ap, CgState* st,
                                                  const double2* __restrict__ red_in, int n_in,
@@ -467,7 +496,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
                           : 0.0;
         if (lead) {
             st->bnorm = bnorm;
-            st->r_rep = r0;
+            st->r_rep[0] = r0;
             st->floor_rel = bnorm > 0.0 ? floor0 / bnorm : floor0;
         }
         if (r0 < tol || (bnorm > 0.0 && rel < tol) || (mode != M_REF && r0 <= floor0)) {
@@ -485,14 +514,25 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
         const double res = sqrt(rr);
         rel = bnorm > 0.0 ? res / bnorm : res;  // cg.rs:118-119
         const bool fresh = mode != M_REF && st->rep_mark == j;
-        // a fresh (true) residual at the FP64 rounding floor of x is as small
-        // as any FP64 solution's: converged to attainable accuracy
-        const double floorr =
-            fresh ? 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh))
-                  : 0.0;
-        if (lead && fresh) st->floor_rel = bnorm > 0.0 ? floorr / bnorm : floorr;
+        const uint64_t nrep = st->n_rep;
+        double floorr = 0.0;
+        bool halved = true;
+        uint32_t nstag = 0;
+        bool done = rel < tol;  // cg.rs:120-122
+        if (fresh) {
+            // r is the true residual (double-double).  Converged when it is
+            // below tol, or at the floor that rounding x to FP64 alone leaves
+            // (no FP64 vector does better), or close to that floor once
+            // replacement cycles stop halving it (the attainable accuracy)
+            floorr = 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh));
+            halved = res <= 0.5 * st->r_rep[(nrep - 1) & 1];
+            nstag = halved ? 0u : st->n_stag[(nrep - 1) & 1] + 1u;
+            done = done || res <= floorr || (!halved && res <= 2.0 * floorr) ||
+                   (nstag >= 3 && res <= 8.0 * floorr);
+            if (lead) st->floor_rel = bnorm > 0.0 ? floorr / bnorm : floorr;
+        }
         if (mode == M_REF || fresh) {
-            if (rel < tol || (fresh && res <= floorr)) {  // cg.rs:120-122
+            if (done) {
                 if (lead) {
                     st->status = ST_CONVERGED;
                     st->iters = j - 1;
@@ -501,12 +541,19 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
                 }
                 return;
             }
-        } else if (rel < tol || res < (low_prev ? st->delta : st->delta64) * st->r_rep) {
-            if (lead) {
-                st->need_rep = 1;  // recompute r as the FP64 true residual first
-                st->r_trig = res;
+        } else {
+            // replacement: the recursive residual reached tol or fell by delta
+            // since the last one, or the cycle runs long (a stagnating
+            // recursion is checked against the true residual)
+            const uint64_t cyc = (j - 1) - st->rep_it;
+            if (rel < tol || res < (low_prev ? st->delta : st->delta64) * st->r_rep[nrep & 1] ||
+                (st->cyc_last > 0 && cyc > 2 * st->cyc_last + 16)) {
+                if (lead) {
+                    st->need_rep = 1;  // recompute r as the true residual first
+                    st->r_trig = res;
+                }
+                return;
             }
-            return;
         }
         if (j > st->max_iter) {
             if (lead) {
@@ -518,13 +565,25 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
             return;
         }
         if (lead) st->rel_last = rel;
-        if (lead && (fresh || (LOW && !low_prev))) {
-            // a replacement cycle on the FP32 operator in which the recursive
-            // residual fell by delta but the true one not even by 2: the FP32
-            // operator's error has reached the residual; finish on FP64 values
-            if (fresh && low_prev && res > 0.5 * st->r_rep) st->stalled = 1;
-            st->r_rep = res;
+        if (lead && fresh) {
+            // a replacement cycle on the FP32 operator that did not halve the
+            // true residual, or took over 1.25x the iterations of the last
+            // FP64 cycle: the FP32 operator's error slows CG down here; the
+            // host finishes the solve on FP64 values
+            const uint64_t cyc = (j - 1) - st->rep_it;
+            if (low_prev && (!halved || (st->cyc_hi > 0 && 4 * cyc > 5 * st->cyc_hi + 8)))
+                st->stalled = 1;
+            if (!low_prev) st->cyc_hi = cyc;
+            st->cyc_last = cyc;
+            st->rep_it = j - 1;
+            st->r_rep[nrep & 1] = res;
+            st->n_stag[nrep & 1] = nstag;
         }
+        // the FP32 operator's cycles are measured from the recursive residual
+        // where it took over.  Blocks of this kernel that read the slot after
+        // this write see a replacement threshold below res: the same "no
+        // replacement" that block 0 decided before writing it.
+        if (lead && LOW && !low_prev && !fresh) st->r_rep[nrep & 1] = res;
         // keep the recurrence across a replacement (van der Vorst & Ye) unless
         // the true residual is far from the recursive one it replaces: then
         // the Krylov information is stale and CG restarts from the true one
@@ -550,33 +609,38 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        double xr[BS], acc[BS];
+        double xr[BS], xz[BS], acc[BS];
 #pragma unroll
         for (int q = 0; q < BS; ++q) {
             const uint64_t i = ((uint64_t)s * BS + q) * SLICE + lane;
-            xr[q] = madd(z[i], beta, pold[i], ORD);
+            xz[q] = z[i];
+            xr[q] = madd(xz[q], beta, pold[i], ORD);
         }
         auto gather = [&](uint32_t c, double* xv) {
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
                 const uint64_t ci = li<BS>(c, q);
-                xv[q] = madd(z[ci], beta, pold[ci], ORD);
+                if constexpr (REC) xv[q] = z[ci];
+                else xv[q] = madd(z[ci], beta, pold[ci], ORD);
             }
         };
+        // own-row entry of the gathered vector (difference form)
+        const double* xo = REC ? xz : xr;
         constexpr int U = BS == 1 ? 4 : 2;
         if constexpr (LOW) {
-            row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xr, gather, acc);
-            add_rowsum<BS>(A.S32, s, lane, xr, acc);
+            row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
+            add_rowsum<BS>(A.S32, s, lane, xo, acc);
         } else if constexpr (ORD) {
             row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
         } else {
-            row_product_pipe<BS, U, double, DIFF, true>(A, A.v64, s, lane, xr, gather, acc);
-            if (DIFF) add_rowsum<BS>(A.S64, s, lane, xr, acc);
+            row_product_pipe<BS, U, double, DIFF, true>(A, A.v64, s, lane, xo, gather, acc);
+            if (DIFF) add_rowsum<BS>(A.S64, s, lane, xo, acc);
         }
         // every lane stores (rows past the last hold zeros): full 512-byte stores
 #pragma unroll
         for (int a = 0; a < BS; ++a) {
             const uint64_t i = ((uint64_t)s * BS + a) * SLICE + lane;
+            if constexpr (REC) acc[a] = fma(beta, Ap[i], acc[a]);
             pnew[i] = xr[a];
             Ap[i] = acc[a];
             dot += xr[a] * acc[a];
@@ -665,7 +729,7 @@ __device__ __forceinline__ double diag_entry(const CgMat& A, uint32_t r, int a, 
         const uint64_t e = (uint64_t)base + (uint64_t)k * SLICE;
         if (A.cols[e + lane] == r) {
             found = true;
-            return A.v64[e * BS * BS + (uint64_t)(a * BS + a) * SLICE + lane];
+            return A.v64[vpos(e, a * BS + a, lane, BS * BS)];
         }
     }
     return 0.0;
@@ -901,7 +965,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     hs->max_iter = args.max_iter;
     hs->delta = env_dbl("FH_RR_DELTA", 1e-1);
     hs->delta64 = env_dbl("FH_RR_DELTA64", 1e-1);
-    hs->sw = env_dbl("FH_MIXED_SWITCH", 1e-5);
+    hs->sw = env_dbl("FH_MIXED_SWITCH", 1e-3);
     hs->mode = mode;
     hs->restart = mode == M_MIXED_IR ? 1 : 0;
     hs->rep_mark = 1;
@@ -928,24 +992,32 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         ev.resize(2 * M);
         for (auto& e : ev) FH_HIP(hipEventCreate(&e));
     }
+    // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
+    // the initial residual or a replacement, where the exact product restarts it
+    const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
     auto launch_chunk = [&](int nin_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m], s));
             const int nin = m == 0 ? nin_first : (int)gu;
+            const bool rec = aprec && !(m == 0 && nin_first == (int)gs);
+#define FH_SPMV(D, L)                                                                          \
+    do {                                                                                       \
+        if (rec)                                                                               \
+            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, true>), dim3(gs), dim3(256), 0, s, A, \
+                               z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);  \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, false>), dim3(gs), dim3(256), 0, s, A, \
+                               z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);  \
+    } while (0)
             if constexpr (DIFF) {
-                if (lowk)
-                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, true, true>), dim3(gs), dim3(256), 0, s,
-                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
-                else if (diff64)
-                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, true, false>), dim3(gs), dim3(256), 0, s,
-                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
-                else
-                    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, false, false>), dim3(gs), dim3(256), 0, s,
-                                       A, z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
+                if (lowk) FH_SPMV(true, true);
+                else if (diff64) FH_SPMV(true, false);
+                else FH_SPMV(false, false);
             } else {
                 hipLaunchKernelGGL((k_cg_spmv<BS, ORD, false, false>), dim3(gs), dim3(256), 0, s, A,
                                    z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
             }
+#undef FH_SPMV
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m + 1], s));
             hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, npad, x, r.p,
                                Ap.p, p0.p, p1.p, dinv.p, z.p, st.p, redA.p, (int)gs, redB.p);

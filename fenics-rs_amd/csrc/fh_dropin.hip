@@ -62,7 +62,7 @@ fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_
             if (k < rlen[r]) {
                 const uint64_t src = row_ptr[r] + k;
                 scols[e + l] = cols[src];
-                for (int j = 0; j < bs2; ++j) svals[e * bs2 + (uint64_t)j * SLICE + l] = vals[src * bs2 + j];
+                for (int j = 0; j < bs2; ++j) svals[vpos(e, j, l, bs2)] = vals[src * bs2 + j];
             } else {
                 scols[e + l] = r;
             }

@@ -8,6 +8,20 @@ namespace fh {
 
 constexpr int SLICE = 64;  // rows per SELL slice == one wavefront
 
+// Position of component j (row-major within the bs x bs block) of the block
+// stored at SELL entry e0 + lane, where e0 = slice_off[slice] + k * SLICE is
+// the entry of lane 0 at step k.  Scalar matrices: e0 + lane.  3x3 blocks: the
+// nine components of a step are 576 consecutive values; components 0-7 are
+// stored as lane pairs (2p, 2p+1) at 128 p + 2 lane, component 8 at 512 + lane,
+// so a wave reads a step as four 16-byte and one 8-byte load per lane, each
+// over a contiguous range.
+__host__ __device__ inline uint64_t vpos(uint64_t e0, int j, uint32_t lane, int bs2) {
+    if (bs2 == 9)
+        return e0 * 9 + (j < 8 ? (uint64_t)(j >> 1) * 128 + 2 * (uint64_t)lane + (j & 1)
+                               : 512 + (uint64_t)lane);
+    return e0 * bs2 + (uint64_t)j * SLICE + lane;
+}
+
 // A symbolic sparsity pattern over "block rows" (a vertex or P2 dof), with the
 // CSR arrays (for export) and the SELL-64 slice layout (for the kernels).
 struct Pattern {

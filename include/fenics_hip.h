@@ -39,7 +39,7 @@ extern "C" {
 /* arithmetic of a solve */
 #define FH_PREC_FP64 0      /* the reference's CG: FP64 SpMV, stop on the recursive residual */
 #define FH_PREC_MIXED 1     /* as FH_PREC_FP64_TRUE, with FP32 matrix values (difference-form
-                               SpMV, FP64 vectors) once ||r||/||b|| < 1e-5                    */
+                               SpMV, FP64 vectors) once ||r||/||b|| < 1e-3                    */
 #define FH_PREC_MIXED_IR 2  /* FP32 matrix values from the start, restarted at each FP64
                                replacement (iterative refinement)                            */
 #define FH_PREC_FP64_TRUE 3 /* FP64 SpMV with periodic replacement by the
