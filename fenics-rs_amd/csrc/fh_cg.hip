@@ -101,7 +101,7 @@ struct CgState {
     double r_rep[2];    // true residual at replacement n (or the recursive one
                         // where the FP32 operator took over), by n_rep parity
     uint32_t n_stag[2]; // consecutive replacement cycles that did not halve it
-    uint32_t pad2_;
+    uint32_t restore;   // 1 + slot of the saved iterate to return (0: the current x)
     double r_trig;      // recursive residual that triggered the pending replacement
     uint64_t n_restart;
     double rel;
@@ -525,16 +525,26 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
             // (no FP64 vector does better), or close to that floor once
             // replacement cycles stop halving it (the attainable accuracy)
             floorr = 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh));
-            halved = res <= 0.5 * st->r_rep[(nrep - 1) & 1];
+            const double prev = st->r_rep[(nrep - 1) & 1];
+            halved = res <= 0.5 * prev;
             nstag = halved ? 0u : st->n_stag[(nrep - 1) & 1] + 1u;
-            done = done || res <= floorr || (!halved && res <= 2.0 * floorr) ||
-                   (nstag >= 3 && res <= 8.0 * floorr);
-            if (lead) st->floor_rel = bnorm > 0.0 ? floorr / bnorm : floorr;
+            const bool stop_stag = (!halved && res <= 2.0 * floorr) || nstag >= 2;
+            done = done || res <= floorr || stop_stag;
+            if (lead) {
+                st->floor_rel = bnorm > 0.0 ? floorr / bnorm : floorr;
+                // near the floor, further cycles can make the true residual
+                // worse: return the iterate of the previous replacement (saved
+                // by the host in slot nrep & 1) when it was better
+                if (done && rel >= tol && res > floorr && nrep >= 2 && res > prev)
+                    st->restore = 1u + (uint32_t)(nrep & 1);
+            }
         }
         if (mode == M_REF || fresh) {
             if (done) {
                 if (lead) {
-                    st->status = ST_CONVERGED;
+                    // stagnation well above the floor is reported, not hidden
+                    const bool ok = rel < tol || res <= 8.0 * floorr || mode == M_REF;
+                    st->status = ok ? ST_CONVERGED : ST_NOT_CONVERGED;
                     st->iters = j - 1;
                     st->res = res;
                     st->rel = rel;
@@ -942,6 +952,12 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
 
     DBuf<double> r(dev, npad), z(dev, npad), p0(dev, npad), p1(dev, npad), Ap(dev, npad);
     DBuf<double> xl(dev, npad), bl(dev, npad);
+    // true-residual modes: the iterate at each of the last two replacements
+    DBuf<double> xsave[2];
+    if (DIFF) {
+        xsave[0].alloc(dev, npad);
+        xsave[1].alloc(dev, npad);
+    }
     DBuf<double> dinv(dev, PRE == 0 ? (size_t)npad : (size_t)nsl * SLICE * 9);
     if (PRE == 1) to_li<9>(args.block_dinv, dinv.p, nrows, nsl, 0.0, s);
     else if (args.inv_diag_override) to_li<BS>(args.inv_diag_override, dinv.p, nrows, nsl, 1.0, s);
@@ -1074,6 +1090,8 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                          (unsigned long long)hs->n_restart, (int)lowk, (int)hs->need_rep,
                          (int)hs->stalled);
         if (hs->need_rep) {
+            // keep the iterate this replacement evaluates (slot = count before k_mark)
+            d2d(xsave[hs->n_rep & 1].p, x, npad, s);
             // residual replacement: r = b - A x (double-double evaluation)
             hipLaunchKernelGGL((k_residual<BS, false, true, PRE>), dim3(gs), dim3(256), 0, s, A,
                                bl.p, x, dinv.p, r.p, z.p, redB.p, (double*)nullptr, fl.p);
@@ -1090,6 +1108,9 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         if (graph2[g]) (void)hipGraphDestroy(graph2[g]);
     }
 
+    FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (DIFF && hs->restore) d2d(x, xsave[hs->restore - 1].p, npad, s);
     // the solution in the caller's layout, and its true residual (double-double)
     hipLaunchKernelGGL(k_from_li<BS>, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0, s, x, args.x,
                        nrows);
