@@ -156,6 +156,43 @@ __global__ __launch_bounds__(256) void k_asm_elasticity(
     }
 }
 
+// mass.rs:86-126: M_{(v,al),(b,al)} += density * vol * (1/10 if a == b else
+// 1/20) for every incident cell, the same scalar mass on each component.
+__global__ __launch_bounds__(256) void k_asm_elasticity_mass(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
+    double density, double* __restrict__ vals) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const RowCtx rc = row_ctx(soff, rlen, v);
+    init_row<9>(vals, rc);
+    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
+        const uint32_t c = aent[e] / 4, a = aent[e] % 4;
+        double x[4][3];
+        uint32_t t[4];
+        load_cell<3, 4>(V, C, c, x, t);
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double vol = (1.0 / 6.0) * fabs(m.det);
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t k = slots[(uint64_t)e * 4 + b];
+            const double factor = (a == b) ? 1.0 / 10.0 : 1.0 / 20.0;
+            const double mv = density * vol * factor;
+#pragma unroll
+            for (int al = 0; al < 3; ++al) vals[vidx(rc.soff, k, al * 3 + al, rc.lane, 9)] += mv;
+        }
+    }
+}
+
+// Dirichlet fold (eigen.rs:51-78): the diagonal value of a constrained dof is
+// `diag` (1 for the stiffness, 0 for the mass), whatever its multiplicity.
+__global__ void k_fold_count(const uint32_t* __restrict__ count, uint32_t n, uint32_t diag,
+                             uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = count[i] ? diag : 0u;
+}
+
 // P2 tetrahedron stiffness, FP64 VALU: K(a,b) = sum_q (w_q |det J|) (g_a^q . g_b^q)
 // with g^q = J^{-T} grad phi(xi_q) (lagrange.rs:324-346 at the
 // quadrature.rs:157-184 points).  G = reference gradients [q][dof][3].
@@ -490,10 +527,24 @@ fh_matrix* assemble_elasticity(fh_mesh* m, double E, double nu, const double* Ec
     return a.release();
 }
 
+fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density) {
+    FH_CHECK(m->dim == 3, "the elasticity mass needs a tetrahedral mesh");
+    DeviceCtx& d = device(m->dev);
+    auto pat = mesh_pattern(m);
+    Adjacency& adj = mesh_adjacency(m);
+    std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 3));
+    hipLaunchKernelGGL(k_asm_elasticity_mass, dim3(cdiv(m->nv, 256)), dim3(256), 0, d.stream,
+                       m->verts.p, m->cells.p, adj.ptr.p, adj.ent.p, adj.slots.p,
+                       pat->slice_off.p, pat->row_len.p, m->nv, density, a->vals.p);
+    FH_LAUNCH_CHECK("asm_elasticity_mass");
+    FH_HIP(hipStreamSynchronize(d.stream));
+    return a.release();
+}
+
 // Condense a copy of `a` (values shared pattern) and lift b (device).
 fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
                            const uint64_t* dofs, const double* vals, double* b_out,
-                           hipStream_t s) {
+                           hipStream_t s, int fold_diag) {
     const int dev = a->dev;
     for (uint64_t k = 0; k < nfix; ++k)
         FH_CHECK(dofs[k] < a->n, "Dirichlet dof out of range");
@@ -521,6 +572,11 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
                            winner.p, o->fixed_count.p);
     hipLaunchKernelGGL(k_bc_values, dim3(cdiv(a->n, 256)), dim3(256), 0, s, winner.p, dvals.p,
                        a->n, o->fixed.p, g.p);
+    if (fold_diag >= 0) {  // Dirichlet fold: fixed diagonal = fold_diag, no multiplicity
+        hipLaunchKernelGGL(k_fold_count, dim3(cdiv(a->n, 256)), dim3(256), 0, s,
+                           o->fixed_count.p, a->n, (uint32_t)fold_diag, o->fixed_count.p);
+        o->folded = true;
+    }
     const Pattern& p = *a->pat;
     const uint64_t nt = (uint64_t)p.nrows * a->bs;
     if (a->bs == 1)
@@ -576,6 +632,7 @@ static HostCsr export_scalar_csr(const fh_matrix* a) {
                     if (a->condensed && (fixed[i] || fixed[c]) && i != c) continue;
                     const uint64_t ix =
                         vpos((uint64_t)soff[s] + (uint64_t)k * SLICE, al * bs + be, l, bs2);
+                    if (a->folded && vals[ix] == 0.0) continue;  // eigen.rs:68
                     h.ci.push_back((uint32_t)c);
                     h.v.push_back(vals[ix]);
                 }

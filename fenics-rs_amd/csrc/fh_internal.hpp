@@ -83,6 +83,7 @@ struct fh_matrix {
     fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
     uint64_t nnz_scalar = 0; // canonical CSC nnz (export)
     bool condensed = false;
+    bool folded = false;     // Dirichlet fold: the export drops every exact zero
     fh::DBuf<uint32_t> fixed_count;  // duplicates in the prescribed list (diag value)
 };
 
@@ -117,9 +118,12 @@ void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ece
                               fh_matrix* a, hipStream_t s);
 void rhs_p1_dev(fh_mesh* m, const double* fq, double* b, hipStream_t s);
 void body_force_dev(fh_mesh* m, const double f[3], double* b, hipStream_t s);
+// fold_diag >= 0: the Dirichlet fold of eigen.rs:51-78 (constrained diagonal =
+// fold_diag, exact zeros dropped on export) instead of the condensation
 fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
                            const uint64_t* dofs, const double* vals, double* b_out,
-                           hipStream_t s);
+                           hipStream_t s, int fold_diag = -1);
+fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density);
 fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_ptr,
                            const uint32_t* cols, const double* vals);
 void ensure_vals32(fh_matrix* a);

@@ -155,6 +155,10 @@ def lib():
         "fh_vertex_von_mises": (C.c_int, [vp, vp, vp]),
         "fh_batched_poisson_block_cg": (C.c_int, [u64, vp, u64, vp, dbl, u64, C.c_int, C.c_int,
                                                   vp, vp, C.POINTER(SolveStats)]),
+        "fh_assemble_elasticity_mass": (C.c_int, [vp, dbl, PP]),
+        "fh_fold_dirichlet": (C.c_int, [vp, u64, vp, C.c_int, PP]),
+        "fh_vibration_modes": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl, dbl, u64, C.c_int,
+                                         vp, vp, vp, vp, C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -352,8 +356,15 @@ class SparseMatrix:
         _check(lib().fh_matrix_info(self.h, C.byref(n), C.byref(nnz), C.byref(bs)))
         return int(n.value), int(nnz.value), bs.value
 
+    @property
+    def n(self):
+        """Number of scalar rows (without counting the exported nonzeros)."""
+        n = u64()
+        _check(lib().fh_matrix_info(self.h, C.byref(n), None, None))
+        return int(n.value)
+
     def nrows(self):
-        return self._info()[0]
+        return self.n
 
     ncols = nrows
 
@@ -760,6 +771,52 @@ def batched_poisson_block_cg(sizes, f_nodal, tol=1e-10, max_iter=20000, precisio
         out.append(u[o:o + k])
         o += k
     return out, its, st
+
+
+def assemble_elasticity_mass_3d(mesh: Mesh, density: float) -> SparseMatrix:
+    """mass.rs:86-126: the 3D vector mass rho * int phi_i phi_j delta_ab."""
+    return _new_matrix(lib().fh_assemble_elasticity_mass, mesh.h, float(density))
+
+
+def fold_dirichlet(mat: SparseMatrix, fixed_dofs, unit_diag: bool) -> SparseMatrix:
+    """eigen.rs:51-78."""
+    d = _u64(fixed_dofs)
+    return _new_matrix(lib().fh_fold_dirichlet, mat.h, len(d), _p(d), 1 if unit_diag else 0)
+
+
+class VibrationMode:
+    """eigen.rs:9-19."""
+
+    def __init__(self, frequency_hz, omega_rad_s, eigenvalue, mode_shape):
+        self.frequency_hz = frequency_hz
+        self.omega_rad_s = omega_rad_s
+        self.eigenvalue = eigenvalue
+        self.mode_shape = mode_shape
+
+
+def solve_vibration_modes(k: SparseMatrix, m: SparseMatrix, fixed_dofs, num_modes: int,
+                          max_power_iters: int, tol: float, inner_tol: float = 1e-9,
+                          inner_max_iter: int = 1000, precision: int = FP64):
+    """eigen.rs:288-300 (and solve_vibration_modes_with_inner, eigen.rs:94-266,
+    with the device Jacobi-CG as the inner solve).  Returns the list of
+    VibrationMode; the inner-solve iteration total is in .inner_iterations of
+    the returned list."""
+    d = _u64(fixed_dofs)
+    n = k.n
+    f, om, ev = np.zeros(num_modes), np.zeros(num_modes), np.zeros(num_modes)
+    shapes = np.zeros((num_modes, n))
+    it = u64(0)
+    _check(lib().fh_vibration_modes(k.h, m.h, len(d), _p(d), num_modes, max_power_iters, tol,
+                                    inner_tol, inner_max_iter, precision, _p(f), _p(om),
+                                    _p(ev), _p(shapes), C.byref(it)))
+    out = _ModeList(VibrationMode(float(f[i]), float(om[i]), float(ev[i]), shapes[i])
+                    for i in range(num_modes))
+    out.inner_iterations = int(it.value)
+    return out
+
+
+class _ModeList(list):
+    inner_iterations = 0
 
 
 def clamp_dofs_x0(vertices, tol=1e-6):

@@ -254,6 +254,27 @@ int fh_batched_poisson_block_cg(uint64_t n_inst, const uint64_t* sizes, uint64_t
                                 int precision, int device, double* u_out,
                                 uint64_t* iters_out, fh_solve_stats* stats);
 
+/* ---------------- modal analysis (the CG's consumer in eigen.rs) ---------------- */
+/* replaces assemble_elasticity_mass_3d (mass.rs:86-126): 3x3-block vector mass
+   M_{(i,a),(j,a)} = density * vol * (1/10 diagonal, 1/20 off) per tetrahedron */
+int fh_assemble_elasticity_mass(fh_mesh* m, double density, fh_matrix** out);
+/* replaces fold_dirichlet (eigen.rs:51-78): constrained rows and columns
+   removed, constrained diagonal 1 (unit_diag) or 0; the CSC export drops exact
+   zeros as the reference does */
+int fh_fold_dirichlet(const fh_matrix* a, uint64_t nfix, const uint64_t* dofs, int unit_diag,
+                      fh_matrix** out);
+/* replaces solve_vibration_modes (eigen.rs:288-300) and, with the device CG
+   as the inner solve, solve_vibration_modes_with_inner (eigen.rs:94-266):
+   inverse power iteration with M-deflation; the inner solve K w = s M v runs
+   from x0 = v to inner_tol in at most inner_max_iter iterations (the
+   reference: 1e-9, 1000) with the given FH_PREC_* precision; shapes
+   [num_modes][n] may be NULL.  Errors of the inner solve are returned. */
+int fh_vibration_modes(const fh_matrix* K, const fh_matrix* M, uint64_t nfix,
+                       const uint64_t* fixed, uint64_t num_modes, uint64_t max_power_iters,
+                       double tol, double inner_tol, uint64_t inner_max_iter, int precision,
+                       double* freq_hz, double* omega, double* eigenvalue, double* shapes,
+                       uint64_t* inner_iterations);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,12 @@ def lib():
                                                C.c_double, C.c_int]),
             "or_assemble_elasticity_3d_cellE": (vp, [C.c_uint64, f64p, C.c_uint64, u64p, f64p,
                                                      C.c_double, C.c_int]),
+            "or_assemble_elasticity_mass_3d": (vp, [C.c_uint64, f64p, C.c_uint64, u64p,
+                                                    C.c_double, C.c_int]),
+            "or_fold_dirichlet": (vp, [vp, C.c_uint64, u64p, C.c_int]),
+            "or_vibration_modes": (C.c_int, [vp, vp, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
+                                             C.c_double, C.c_double, C.c_uint64, f64p, f64p,
+                                             f64p, vp]),
             "or_quad_points_2d": (C.c_int, [f64p, C.c_uint64, u64p, C.c_int, f64p]),
             "or_quad_points_3d": (C.c_int, [f64p, C.c_uint64, u64p, C.c_int, f64p]),
             "or_assemble_rhs_2d": (C.c_int, [C.c_uint64, f64p, C.c_uint64, u64p, f64p, f64p]),
@@ -288,6 +294,29 @@ def assemble_elasticity(m: Mesh, E, nu, nthreads=0) -> Csc:
                                                    float(nu), nthreads))
     return Csc(lib().or_assemble_elasticity_3d_cellE(m.nv, m.vertices, m.nc, m.cells, _f64(E),
                                                      float(nu), nthreads))
+
+
+def assemble_elasticity_mass_3d(m: Mesh, density, nthreads=0) -> Csc:
+    """mass.rs:86-126."""
+    return Csc(lib().or_assemble_elasticity_mass_3d(m.nv, m.vertices, m.nc, m.cells,
+                                                    float(density), nthreads))
+
+
+def fold_dirichlet(A: Csc, fixed, unit_diag: bool) -> Csc:
+    """eigen.rs:51-78."""
+    return Csc(lib().or_fold_dirichlet(A.h, len(fixed), _u64(fixed), 1 if unit_diag else 0))
+
+
+def vibration_modes(K: Csc, M: Csc, fixed, num_modes, max_power_iters, tol, inner_tol=1e-9,
+                    inner_max_iter=1000, shapes=True):
+    """eigen.rs:94-300 with the Jacobi-CG inner solve.  Returns (status,
+    freq_hz, omega, eigenvalue, shapes [num_modes, n] or None)."""
+    f, om, ev = np.zeros(num_modes), np.zeros(num_modes), np.zeros(num_modes)
+    sh = np.zeros((num_modes, K.n)) if shapes else None
+    st = lib().or_vibration_modes(K.h, M.h, len(fixed), _u64(fixed), num_modes, max_power_iters,
+                                  tol, inner_tol, inner_max_iter, f, om, ev,
+                                  sh.ctypes.data if shapes else None)
+    return st, f, om, ev, sh
 
 
 def quad_points(m: Mesh, degree):

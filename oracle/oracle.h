@@ -99,6 +99,23 @@ double or_l2_error_2d(const double* verts, uint64_t nc, const uint64_t* cells, c
 double or_l2_error_3d(const double* verts, uint64_t nc, const uint64_t* cells, const double* u,
                       const double* uq);
 
+/* mass.rs:86-126 (3D vector mass for elasticity, 3*nv dofs) */
+or_csc* or_assemble_elasticity_mass_3d(uint64_t nv, const double* verts, uint64_t nc,
+                                       const uint64_t* cells, double density, int nthreads);
+
+/* ---------------- modal analysis: eigen.rs ---------------- */
+/* eigen.rs:51-78: drop entries on constrained rows/columns and exact zeros;
+   unit_diag: constrained diagonal = 1 */
+or_csc* or_fold_dirichlet(const or_csc* a, uint64_t nfix, const uint64_t* fixed, int unit_diag);
+/* eigen.rs:94-266 with the inner solve solve_cg(kd, b, x0, inner_tol,
+   inner_max_iter) (eigen.rs:288-300 uses 1e-9, 1000).  shapes: [num_modes][n]
+   or NULL.  Returns 0, or the inner CG's status on its failure (the Err of
+   eigen.rs), or 4 when no dof is free. */
+int or_vibration_modes(const or_csc* k, const or_csc* m, uint64_t nfix, const uint64_t* fixed,
+                       uint64_t num_modes, uint64_t max_power_iters, double tol,
+                       double inner_tol, uint64_t inner_max_iter, double* freq_hz,
+                       double* omega, double* eigenvalue, double* shapes);
+
 /* ---------------- solver: cg.rs, gpu.rs CSC->CSR ---------------- */
 void or_spmv(const or_csc* a, const double* x, double* y);
 int or_solve_cg(const or_csc* a, const double* b, const double* x0, double tol,

@@ -252,6 +252,26 @@ or_csc* or_assemble_mass_2d(uint64_t nv, const double* V, uint64_t nc, const uin
     return csc_from_trips(nv, tr);
 }
 
+// mass.rs:86-126: M_{(i,a),(j,a)} = density * vol * (1/10 if i == j else 1/20),
+// the same scalar mass on each displacement component.
+or_csc* or_assemble_elasticity_mass_3d(uint64_t nv, const double* V, uint64_t nc,
+                                       const uint64_t* C, double density, int nthreads) {
+    auto tr = gen_trips(nc, nthreads, 48, [&](uint64_t c, std::vector<Trip>& out) {
+        const uint64_t* t = C + 4 * c;
+        Map3 m;
+        if (!affine3(V + 3 * t[0], V + 3 * t[1], V + 3 * t[2], V + 3 * t[3], m)) return;
+        const double vol = (1.0 / 6.0) * std::fabs(m.det);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                const double factor = (i == j) ? 1.0 / 10.0 : 1.0 / 20.0;
+                const double mv = density * vol * factor;
+                for (int a = 0; a < 3; ++a)
+                    out.push_back({(uint32_t)(3 * t[i] + a), (uint32_t)(3 * t[j] + a), mv});
+            }
+    });
+    return csc_from_trips(3 * nv, tr);
+}
+
 // elasticity.rs:73-149.
 or_csc* or_assemble_elasticity_3d(uint64_t nv, const double* V, uint64_t nc, const uint64_t* C,
                                   double E, double nu, int nthreads) {

@@ -52,11 +52,135 @@ std::vector<double> inv_diagonal(const or_csc* A) {
     return d;
 }
 
+// eigen.rs:22-45: mean |diagonal| over the columns that store a diagonal entry.
+double mean_abs_diag(const or_csc* A) {
+    if (A->n == 0) return 0.0;
+    double acc = 0.0;
+    uint64_t found = 0;
+    for (uint64_t c = 0; c < A->n; ++c)
+        for (uint64_t p = A->col_ptr[c]; p < A->col_ptr[c + 1]; ++p)
+            if (A->row_idx[p] == c) {
+                acc += std::fabs(A->vals[p]);
+                ++found;
+                break;
+            }
+    return found ? acc / (double)found : 0.0;
+}
+
 }  // namespace
 
 extern "C" {
 
 void or_spmv(const or_csc* a, const double* x, double* y) { spmv(a, x, y); }
+
+// eigen.rs:51-78, column by column: drop constrained rows/columns and exact
+// zeros; append the unit diagonal of a constrained column after its entries.
+or_csc* or_fold_dirichlet(const or_csc* a, uint64_t nfix, const uint64_t* fixed, int unit_diag) {
+    const uint64_t n = a->n;
+    std::vector<char> isf(n, 0);
+    for (uint64_t i = 0; i < nfix; ++i)
+        if (fixed[i] < n) isf[fixed[i]] = 1;
+    std::vector<Trip> tr;
+    for (uint64_t c = 0; c < n; ++c) {
+        for (uint64_t p = a->col_ptr[c]; p < a->col_ptr[c + 1]; ++p) {
+            const uint64_t r = a->row_idx[p];
+            if (isf[r] || isf[c] || a->vals[p] == 0.0) continue;
+            tr.push_back({(uint32_t)r, (uint32_t)c, a->vals[p]});
+        }
+        if (unit_diag && isf[c]) tr.push_back({(uint32_t)c, (uint32_t)c, 1.0});
+    }
+    return csc_from_trips(n, tr);
+}
+
+// eigen.rs:94-266: inverse power iteration with M-deflation against the
+// converged modes; mass rescaled by s = mean|diag K| / mean|diag M| (exact:
+// eigenvalues are recovered with the unscaled M).
+int or_vibration_modes(const or_csc* k, const or_csc* m, uint64_t nfix, const uint64_t* fixed,
+                       uint64_t num_modes, uint64_t max_power_iters, double tol,
+                       double inner_tol, uint64_t inner_max_iter, double* freq_hz,
+                       double* omega_out, double* eig_out, double* shapes) {
+    or_csc* kd = or_fold_dirichlet(k, nfix, fixed, 1);
+    or_csc* md = or_fold_dirichlet(m, nfix, fixed, 0);
+    const uint64_t n = kd->n;
+    std::vector<char> isf(n, 0);
+    for (uint64_t i = 0; i < nfix; ++i)
+        if (fixed[i] < n) isf[fixed[i]] = 1;
+    uint64_t nfree = 0;
+    for (uint64_t i = 0; i < n; ++i) nfree += isf[i] ? 0 : 1;
+    int code = 0;
+    if (nfree == 0) code = 4;
+    const double kdd = mean_abs_diag(kd), mdd = mean_abs_diag(md);
+    const double s = (kdd > 0.0 && mdd > 0.0) ? kdd / mdd : 1.0;
+    std::vector<std::vector<double>> modes;
+    std::vector<double> v(n), mv(n), w(n), kw(n), mw(n);
+    const double pi = 3.14159265358979323846;
+    for (uint64_t mi = 0; mi < num_modes && code == 0; ++mi) {
+        for (uint64_t i = 0; i < n; ++i) {
+            v[i] = 0.0;
+            if (isf[i]) continue;
+            const uint64_t q = ((i + 1) * 31 + (mi + 1) * 97) % 1000;
+            v[i] = (double)q / 1000.0 - 0.5;
+        }
+        std::vector<std::vector<double>> mprev;
+        for (const auto& p : modes) {
+            std::vector<double> y(n);
+            spmv(md, p.data(), y.data());
+            for (auto& t : y) t *= s;
+            mprev.push_back(std::move(y));
+        }
+        for (size_t q = 0; q < modes.size(); ++q) {
+            const double coeff = dot(v.data(), mprev[q].data(), n);
+            for (uint64_t i = 0; i < n; ++i) v[i] -= coeff * modes[q][i];
+        }
+        spmv(md, v.data(), mv.data());
+        for (auto& t : mv) t *= s;
+        const double norm_m = std::sqrt(dot(v.data(), mv.data(), n));
+        if (norm_m > 1e-14)
+            for (auto& t : v) t /= norm_m;
+        double prev = 0.0;
+        for (uint64_t it = 0; it < max_power_iters; ++it) {
+            spmv(md, v.data(), mv.data());
+            for (auto& t : mv) t *= s;
+            uint64_t iters = 0;
+            double res = 0.0;
+            const int st = or_solve_cg(kd, mv.data(), v.data(), inner_tol, inner_max_iter,
+                                       w.data(), &iters, &res);
+            if (st != 0) {
+                code = st;
+                break;
+            }
+            for (uint64_t i = 0; i < n; ++i)
+                if (isf[i]) w[i] = 0.0;
+            for (size_t q = 0; q < modes.size(); ++q) {
+                const double coeff = dot(w.data(), mprev[q].data(), n);
+                for (uint64_t i = 0; i < n; ++i) w[i] -= coeff * modes[q][i];
+            }
+            spmv(md, w.data(), mw.data());
+            const double mwn = std::sqrt(dot(w.data(), mw.data(), n)) * std::sqrt(s);
+            if (mwn < 1e-15) break;
+            for (auto& t : w) t /= mwn;
+            spmv(kd, w.data(), kw.data());
+            const double ray = dot(w.data(), kw.data(), n);
+            v = w;
+            const double denom = std::max(std::fabs(ray), std::fabs(prev));
+            if (denom == 0.0 || std::fabs(ray - prev) / denom < tol) break;
+            prev = ray;
+        }
+        if (code) break;
+        spmv(kd, v.data(), kw.data());
+        spmv(md, v.data(), mw.data());
+        const double lam = std::max(dot(v.data(), kw.data(), n) / dot(v.data(), mw.data(), n), 0.0);
+        const double om = std::sqrt(lam);
+        freq_hz[mi] = om / (2.0 * pi);
+        omega_out[mi] = om;
+        eig_out[mi] = lam;
+        if (shapes) std::copy(v.begin(), v.end(), shapes + mi * n);
+        modes.push_back(v);
+    }
+    or_csc_free(kd);
+    or_csc_free(md);
+    return code;
+}
 
 // cg.rs:40-149, control flow and stopping rule kept exactly:
 // early exit when ||r0|| < tol or ||r0||/||b|| < tol; |pAp| < 1e-30 is a

@@ -1,0 +1,93 @@
+"""GPU modal analysis (the CG's consumer in eigen.rs) against the oracle and
+the reference's modal tests (modal_tests.rs).
+
+Tolerances: the elasticity mass and the Dirichlet folds are assembled with
+the oracle's arithmetic, so their CSC exports are bitwise equal.  The power
+iteration runs the reference's sequence on the device; dot products are tree
+sums and the inner CG differs from cg.rs in its dot order, so eigenvalues from
+tightly converged runs (tol 1e-12, inner 1e-12) agree with the oracle's to
+1e-9 relative and mode shapes to 1e-6 (up to sign); loosely converged runs
+(tol 1e-5, the reference's test) are compared through the reference's own
+frequency ranges and to the oracle at 1e-4."""
+import numpy as np
+import pytest
+
+import fenics_hip as fh
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def beam(nx, ny, nz, rho=1000.0):
+    gm, om = fh.box_beam(1.0, 0.1, 0.1, nx, ny, nz), O.box_beam(1.0, 0.1, 0.1, nx, ny, nz)
+    K = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(1.0e7, 0.3))
+    M = fh.assemble_elasticity_mass_3d(gm, rho)
+    Ko = O.assemble_elasticity(om, 1.0e7, 0.3)
+    Mo = O.assemble_elasticity_mass_3d(om, rho)
+    fixed = np.array([d for d, _ in fh.clamp_dofs_x0(gm.vertices).prescribed_dofs], np.uint64)
+    return gm, om, K, M, Ko, Mo, fixed
+
+
+def same_csc(a, b):
+    c1, r1, v1 = a.export_csc()
+    c2, r2, v2 = b.export()
+    return np.array_equal(c1, c2) and np.array_equal(r1, r2) and np.array_equal(bits(v1), bits(v2))
+
+
+def test_elasticity_mass_and_folds_bitwise():
+    gm, om, K, M, Ko, Mo, fixed = beam(4, 2, 2, rho=7.5)
+    assert same_csc(M, Mo)
+    assert same_csc(fh.fold_dirichlet(K, fixed, True), O.fold_dirichlet(Ko, fixed, True))
+    assert same_csc(fh.fold_dirichlet(M, fixed, False), O.fold_dirichlet(Mo, fixed, False))
+    # duplicates and out-of-range indices (eigen.rs:57-62) are harmless
+    dup = np.concatenate([fixed, fixed[:5], [10 ** 9]]).astype(np.uint64)
+    assert same_csc(fh.fold_dirichlet(K, dup, True), O.fold_dirichlet(Ko, fixed, True))
+
+
+def test_cantilever_natural_frequencies():  # modal_tests.rs:9-86
+    gm, om, K, M, Ko, Mo, fixed = beam(16, 2, 2)
+    modes = fh.solve_vibration_modes(K, M, fixed, 3, 30, 1e-5)
+    f = np.array([md.frequency_hz for md in modes])
+    assert 1.0 < f[0] < 3.0 and 1.5 < f[1] < 3.5 and 8.0 < f[2] < 20.0
+    st, fo, _, _, _ = O.vibration_modes(Ko, Mo, fixed, 3, 30, 1e-5)
+    assert st == 0 and np.allclose(f, fo, rtol=1e-4, atol=0)
+    for md in modes:
+        assert abs(md.omega_rad_s - 2 * np.pi * md.frequency_hz) < 1e-12 * md.omega_rad_s
+        assert not md.mode_shape[fixed].any()
+
+
+def test_frequencies_scale_as_one_over_sqrt_density():  # modal_tests.rs:88-113
+    gm, om, K, M1, Ko, Mo, fixed = beam(8, 1, 1)
+    M2 = fh.assemble_elasticity_mass_3d(gm, 1000.0 * 10000.0)
+    a = fh.solve_vibration_modes(K, M1, fixed, 2, 60, 1e-8)
+    b = fh.solve_vibration_modes(K, M2, fixed, 2, 60, 1e-8)
+    for hi, lo in zip(b, a):
+        assert abs(hi.frequency_hz / lo.frequency_hz - 0.01) < 1e-4
+
+
+def test_tight_modes_match_oracle():
+    gm, om, K, M, Ko, Mo, fixed = beam(12, 2, 2)
+    modes = fh.solve_vibration_modes(K, M, fixed, 3, 300, 1e-13, inner_tol=1e-12,
+                                     inner_max_iter=5000)
+    st, fo, omo, evo, sho = O.vibration_modes(Ko, Mo, fixed, 3, 300, 1e-13, inner_tol=1e-12,
+                                              inner_max_iter=5000)
+    assert st == 0
+    ev = np.array([md.eigenvalue for md in modes])
+    assert np.allclose(ev, evo, rtol=1e-9, atol=0)
+    for md, so in zip(modes, sho):
+        sgn = np.sign(md.mode_shape @ so)
+        assert np.linalg.norm(sgn * md.mode_shape - so) < 1e-6 * np.linalg.norm(so)
+    assert modes.inner_iterations > 0
+
+
+def test_inner_solve_failure_is_reported():
+    """eigen.rs:190 propagates the inner solve's Err: an iteration cap that CG
+    cannot meet makes the whole call fail with the CG's status."""
+    gm, om, K, M, Ko, Mo, fixed = beam(16, 2, 2)
+    with pytest.raises(fh.FenicsError) as e:
+        fh.solve_vibration_modes(K, M, fixed, 1, 5, 1e-8, inner_tol=1e-14, inner_max_iter=3)
+    assert e.value.code == fh.FH_ERR_NOT_CONVERGED

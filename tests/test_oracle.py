@@ -464,3 +464,78 @@ def test_golden_fixtures_match_oracle():
         idx = (3 * cell[:, None] + np.arange(3)).reshape(-1)
         Kd[np.ix_(idx, idx)] += Ke
     assert np.abs(K.to_scipy().toarray() - Kd).max() < 1e-9 * np.abs(Kd).max()
+
+
+# ---------------------------------------------------------------- modal analysis
+def _cantilever_km(nx, ny, nz, rho=1000.0):
+    m = O.box_beam(1.0, 0.1, 0.1, nx, ny, nz)
+    return m, O.assemble_elasticity(m, 1.0e7, 0.3), O.assemble_elasticity_mass_3d(m, rho)
+
+
+def test_elasticity_mass_3d_invariants():
+    """mass.rs:86-126: sum of all entries = 3 rho V (three components), each
+    component block equals the scalar P1 mass (vol/10 diagonal, vol/20 off),
+    symmetric, no coupling between components."""
+    m, _, M = _cantilever_km(4, 2, 2, rho=7.5)
+    S = M.to_scipy().tocoo()
+    assert abs(S.sum() - 3 * 7.5 * 0.01) < 1e-15 * 3 * 7.5
+    assert np.all(S.row % 3 == S.col % 3)
+    D = M.to_scipy().toarray()
+    assert np.array_equal(D, D.T)
+    # one tetrahedron by hand
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    t = O.Mesh(V, np.array([[0, 1, 2, 3]], np.uint64))
+    Mt = O.assemble_elasticity_mass_3d(t, 2.0).to_scipy().toarray()
+    vol = 1.0 / 6.0
+    assert abs(Mt[0, 0] - 2.0 * vol / 10.0) < 1e-17 and abs(Mt[0, 3] - 2.0 * vol / 20.0) < 1e-17
+    assert Mt[0, 1] == 0.0 and Mt[0, 4] == 0.0
+
+
+def test_fold_dirichlet_definition():
+    """eigen.rs:51-78: constrained rows/columns dropped, unit diagonal when asked."""
+    m, K, M = _cantilever_km(3, 1, 1)
+    fx = O.clamp_dofs_x0(m)
+    Kd = O.fold_dirichlet(K, fx, True).to_scipy().toarray()
+    Md = O.fold_dirichlet(M, fx, False).to_scipy().toarray()
+    Kf, Mf = K.to_scipy().toarray(), M.to_scipy().toarray()
+    free = np.setdiff1d(np.arange(K.n), fx)
+    assert np.array_equal(Kd[np.ix_(free, free)], Kf[np.ix_(free, free)])
+    assert np.array_equal(Kd[np.ix_(fx, fx)], np.eye(len(fx)))
+    assert not Kd[np.ix_(fx, free)].any() and not Md[fx].any() and not Md[:, fx].any()
+
+
+def test_cantilever_natural_frequencies():  # modal_tests.rs:9-86
+    m, K, M = _cantilever_km(16, 2, 2)
+    st, f, om, ev, _ = O.vibration_modes(K, M, O.clamp_dofs_x0(m), 3, 30, 1e-5)
+    assert st == 0
+    assert 1.0 < f[0] < 3.0 and 1.5 < f[1] < 3.5 and 8.0 < f[2] < 20.0
+    assert np.allclose(om, 2 * np.pi * f) and np.allclose(ev, om ** 2)
+
+
+def test_frequencies_scale_as_one_over_sqrt_density():  # modal_tests.rs:88-113
+    m, K, M1 = _cantilever_km(8, 1, 1)
+    M2 = O.assemble_elasticity_mass_3d(m, 1000.0 * 10000.0)
+    fx = O.clamp_dofs_x0(m)
+    a = O.vibration_modes(K, M1, fx, 2, 60, 1e-8)[1]
+    b = O.vibration_modes(K, M2, fx, 2, 60, 1e-8)[1]
+    assert np.allclose(b / a, 0.01, rtol=0, atol=1e-4)
+
+
+def test_vibration_modes_vs_shift_invert_lanczos():
+    """Independent pin: scipy's shift-invert eigsh on the free-dof generalized
+    problem K v = lambda M v gives the same lowest eigenvalues (1e-9)."""
+    import scipy.sparse.linalg as sla
+    m, K, M = _cantilever_km(12, 2, 2)
+    fx = O.clamp_dofs_x0(m)
+    st, f, om, ev, sh = O.vibration_modes(K, M, fx, 3, 300, 1e-13, inner_tol=1e-12,
+                                          inner_max_iter=5000)
+    free = np.setdiff1d(np.arange(K.n), fx)
+    Kf = K.to_scipy().tocsc()[free][:, free]
+    Mf = M.to_scipy().tocsc()[free][:, free]
+    lam = np.sort(sla.eigsh(Kf, k=3, M=Mf, sigma=0, which="LM")[0])
+    assert st == 0 and np.allclose(ev, lam, rtol=1e-9, atol=0)
+    # modes are M-orthogonal with one common norm (normalised in the rescaled
+    # mass s M, eigen.rs:172-181, so v^T M v = 1/s) and vanish on the clamped dofs
+    Ms = M.to_scipy()
+    G = sh @ (Ms @ sh.T)
+    assert np.allclose(G / G[0, 0], np.eye(3), atol=1e-6) and not sh[:, fx].any()
