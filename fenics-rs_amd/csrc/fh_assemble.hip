@@ -538,6 +538,8 @@ fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density) {
                        pat->slice_off.p, pat->row_len.p, m->nv, density, a->vals.p);
     FH_LAUNCH_CHECK("asm_elasticity_mass");
     FH_HIP(hipStreamSynchronize(d.stream));
+    a->comp_diag = true;
+    a->nnz_scalar = a->pat->nnzb * 3;  // one entry per component of each block
     return a.release();
 }
 
@@ -561,6 +563,7 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
     o->n = a->n;
     o->nnz_scalar = 0;
     o->condensed = true;
+    o->comp_diag = a->comp_diag;
     o->vals.alloc(dev, a->vals.n);
     d2d(o->vals.p, a->vals.p, a->vals.n, s);
     o->fixed.alloc(dev, a->n);
@@ -595,12 +598,7 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
 
 // Host export to faer's canonical CSC raw parts (optionally dropping the
 // condensed couplings).  Not on the hot path.
-struct HostCsr {
-    std::vector<uint64_t> rp;
-    std::vector<uint32_t> ci;
-    std::vector<double> v;
-};
-static HostCsr export_scalar_csr(const fh_matrix* a) {
+HostCsr export_scalar_csr(const fh_matrix* a) {
     DeviceCtx& d = device(a->dev);
     const Pattern& p = *a->pat;
     const int bs = a->bs, bs2 = bs * bs;
@@ -628,6 +626,7 @@ static HostCsr export_scalar_csr(const fh_matrix* a) {
             for (uint32_t k = 0; k < len; ++k) {
                 const uint32_t cb = cols[rptr[r] + k];
                 for (int be = 0; be < bs; ++be) {
+                    if (a->comp_diag && be != al) continue;  // mass.rs:115-118
                     const uint64_t c = (uint64_t)cb * bs + be;
                     if (a->condensed && (fixed[i] || fixed[c]) && i != c) continue;
                     const uint64_t ix =

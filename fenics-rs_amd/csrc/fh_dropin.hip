@@ -90,6 +90,32 @@ fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_
     return a;
 }
 
+// faer CSC raw parts -> device SELL-64 (gpu.rs:1552-1583: rows filled in
+// ascending column order, so every row's columns stay sorted).
+fh_matrix* matrix_from_csc(int dev, uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
+                           const double* vals) {
+    FH_CHECK(col_ptr, "null CSC arrays");
+    FH_CHECK(n < 0xffffffffull, "matrix too large");
+    FH_CHECK(col_ptr[0] == 0, "col_ptr[0] must be 0");
+    const uint64_t nnz = col_ptr[n];
+    FH_CHECK(nnz < 0xffffffffull, "too many nonzeros");
+    FH_CHECK(nnz == 0 || (row_idx && vals), "null CSC arrays");
+    for (uint64_t c = 0; c < n; ++c) FH_CHECK(col_ptr[c + 1] >= col_ptr[c], "col_ptr not monotone");
+    for (uint64_t i = 0; i < nnz; ++i) FH_CHECK(row_idx[i] < n, "row index out of range");
+    std::vector<uint32_t> rp(n + 1, 0), ci(nnz);
+    std::vector<double> v(nnz);
+    for (uint64_t i = 0; i < nnz; ++i) rp[row_idx[i] + 1]++;
+    for (uint64_t r = 0; r < n; ++r) rp[r + 1] += rp[r];
+    std::vector<uint32_t> cur(rp.begin(), rp.end() - 1);
+    for (uint64_t c = 0; c < n; ++c)
+        for (uint64_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) {
+            const uint32_t q = cur[row_idx[p]]++;
+            ci[q] = (uint32_t)c;
+            v[q] = vals[p];
+        }
+    return matrix_from_csr(dev, (uint32_t)n, 1, rp.data(), ci.data(), v.data());
+}
+
 struct MatPtr {
     fh_matrix* p = nullptr;
     ~MatPtr() { delete p; }
@@ -169,30 +195,19 @@ int fh_solve_csc_cg(uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx
     int code = FH_OK;
     const int g = guard([&] {
         FH_CHECK(col_ptr && b && x_out, "null argument");
-        FH_CHECK(n < 0xffffffffull, "matrix too large");
-        FH_CHECK(col_ptr[0] == 0, "col_ptr[0] must be 0");
-        const uint64_t nnz = col_ptr[n];
-        FH_CHECK(nnz < 0xffffffffull, "too many nonzeros");
-        for (uint64_t c = 0; c < n; ++c) FH_CHECK(col_ptr[c + 1] >= col_ptr[c], "col_ptr not monotone");
-        for (uint64_t i = 0; i < nnz; ++i) FH_CHECK(row_idx[i] < n, "row index out of range");
-        // gpu.rs:1552-1583: CSC -> CSR (rows filled in ascending column order)
-        std::vector<uint32_t> rp(n + 1, 0), ci(nnz);
-        std::vector<double> v(nnz);
-        for (uint64_t i = 0; i < nnz; ++i) rp[row_idx[i] + 1]++;
-        for (uint64_t r = 0; r < n; ++r) rp[r + 1] += rp[r];
-        std::vector<uint32_t> cur(rp.begin(), rp.end() - 1);
-        for (uint64_t c = 0; c < n; ++c)
-            for (uint64_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) {
-                const uint32_t q = cur[row_idx[p]]++;
-                ci[q] = (uint32_t)c;
-                v[q] = vals[p];
-            }
-        const int dev = resolve_device(device_idx);
-        MatPtr a{matrix_from_csr(dev, (uint32_t)n, 1, rp.data(), ci.data(), v.data())};
+        MatPtr a{matrix_from_csc(resolve_device(device_idx), n, col_ptr, row_idx, vals)};
         code = solve_host(a.p, b, x0, tol, max_iter, FH_PREC_FP64, x_out, stats, nullptr, nullptr);
         report(code);
     });
     return g != FH_OK ? g : code;
+}
+
+int fh_matrix_from_csc(uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
+                       const double* vals, int device_idx, fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(out && col_ptr, "null argument");
+        *out = matrix_from_csc(resolve_device(device_idx), n, col_ptr, row_idx, vals);
+    });
 }
 
 int fh_solve_csr_cg(uint64_t n, const uint32_t* row_ptr, const uint32_t* col_idx,

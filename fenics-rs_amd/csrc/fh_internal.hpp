@@ -84,6 +84,8 @@ struct fh_matrix {
     uint64_t nnz_scalar = 0; // canonical CSC nnz (export)
     bool condensed = false;
     bool folded = false;     // Dirichlet fold: the export drops every exact zero
+    bool comp_diag = false;  // blocks couple equal components only (elasticity mass):
+                             // the export leaves out the off-diagonal components
     fh::DBuf<uint32_t> fixed_count;  // duplicates in the prescribed list (diag value)
 };
 
@@ -126,7 +128,17 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
 fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density);
 fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_ptr,
                            const uint32_t* cols, const double* vals);
+fh_matrix* matrix_from_csc(int dev, uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
+                           const double* vals);
 void ensure_vals32(fh_matrix* a);
+// scalar CSR of the operator the matrix exports (condensed couplings, folded
+// zeros and the off-diagonal components of a component-diagonal mass left out)
+struct HostCsr {
+    std::vector<uint64_t> rp;
+    std::vector<uint32_t> ci;
+    std::vector<double> v;
+};
+HostCsr export_scalar_csr(const fh_matrix* a);
 void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s);
 uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
 

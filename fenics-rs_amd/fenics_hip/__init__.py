@@ -159,6 +159,13 @@ def lib():
         "fh_fold_dirichlet": (C.c_int, [vp, u64, vp, C.c_int, PP]),
         "fh_vibration_modes": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl, dbl, u64, C.c_int,
                                          vp, vp, vp, vp, C.POINTER(u64)]),
+        "fh_matrix_from_csc": (C.c_int, [u64, vp, vp, vp, C.c_int, PP]),
+        "fh_add_scaled_matrices": (C.c_int, [vp, vp, dbl, PP]),
+        "fh_transient_heat_step": (C.c_int, [vp, vp, C.c_int, dbl, vp, vp, vp, dbl, u64,
+                                             C.c_int, vp, C.POINTER(SolveStats)]),
+        "fh_solve_transient_heat_2d": (C.c_int, [vp, dbl, vp, u64, vp, vp, dbl, u64, C.c_int,
+                                                 HeatSourceFn, vp, dbl, u64, C.c_int, vp, vp,
+                                                 C.POINTER(TransientStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -817,6 +824,114 @@ def solve_vibration_modes(k: SparseMatrix, m: SparseMatrix, fixed_dofs, num_mode
 
 class _ModeList(list):
     inner_iterations = 0
+
+
+# ------------------------------------------------------- transient heat
+class TimeSteppingScheme:
+    """transient.rs:12-18."""
+    BackwardEuler = 0
+    CrankNicolson = 1
+
+
+class TransientStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("cg_iterations", C.c_uint64),
+                ("cg_iterations_low", C.c_uint64), ("cg_iterations_max", C.c_uint64),
+                ("solve_ms", C.c_double), ("wall_ms", C.c_double), ("device", C.c_int),
+                ("precision", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+HeatSourceFn = C.CFUNCTYPE(C.c_double, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
+def matrix_from_csc(col_ptr, row_idx, values, device: int = -1) -> SparseMatrix:
+    """A faer SparseColMat's raw parts (column pointers, row indices, values)
+    as a device operator."""
+    cp, ri, va = _u64(col_ptr), _u64(row_idx), _f64(values)
+    return _new_matrix(lib().fh_matrix_from_csc, len(cp) - 1, _p(cp), _p(ri), _p(va), device)
+
+
+def add_scaled_matrices(a: SparseMatrix, b: SparseMatrix, alpha: float) -> SparseMatrix:
+    """transient.rs:72-97: a + alpha * b."""
+    return _new_matrix(lib().fh_add_scaled_matrices, a.h, b.h, float(alpha))
+
+
+class TransientHeatSolver:
+    """transient.rs:20-69.  The implicit system is solved by the device CG
+    (from x0 = u_n, to ||r||/||b|| < tol) in place of the reference's dense LU."""
+
+    def __init__(self, scheme: int, dt: float, tol: float = 1e-12, max_iter: int = 10000,
+                 precision: int = FP64_TRUE):
+        self.scheme, self.dt = scheme, dt
+        self.tol, self.max_iter, self.precision = tol, max_iter, precision
+        self.last_stats = None
+
+    def step(self, mass_mat: SparseMatrix, stiff_mat: SparseMatrix, u_n, f_n, f_next):
+        n = mass_mat.n
+        u, fa, fb = _f64(u_n), _f64(f_n), _f64(f_next)
+        if not (len(u) == len(fa) == len(fb) == n):
+            raise FenicsError(FH_ERR_BAD_INPUT, f"vector sizes {len(u)}/{len(fa)}/{len(fb)} "
+                                                f"do not match the matrix dimension {n}")
+        out = np.empty(n)
+        st = SolveStats()
+        _check(lib().fh_transient_heat_step(mass_mat.h, stiff_mat.h, self.scheme, self.dt,
+                                            _p(u), _p(fa), _p(fb), self.tol, self.max_iter,
+                                            self.precision, _p(out), C.byref(st)))
+        self.last_stats = st
+        return out
+
+
+@dataclasses.dataclass
+class TransientHeatSolution:
+    """heat_solver.rs:13-19 (plus the device statistics)."""
+    time_steps: list
+    temperature_history: list
+    final_temperature: np.ndarray
+    stats: dict
+
+
+def solve_transient_heat_2d(mesh: Mesh, diffusivity: float, u_initial, bcs: Sequence[DirichletBC],
+                            dt: float, num_steps: int, scheme: int,
+                            heat_source: Optional[Callable] = None, tol: float = 1e-12,
+                            max_iter: int = 10000, precision: int = FP64_TRUE,
+                            keep_history: bool = True) -> TransientHeatSolution:
+    """heat_solver.rs:23-108.  heat_source(t, point) -> float is sampled at the
+    vertices at t_n and t_{n+1} of every step (None: no source)."""
+    n = mesh.num_vertices()
+    u0 = _f64(u_initial)
+    if len(u0) != n:
+        raise FenicsError(FH_ERR_BAD_INPUT,
+                          f"Initial temperature size {len(u0)} != mesh vertices {n}")
+    dofs = [int(d) for bc in bcs for d in bc.dofs]
+    vals = [float(v) for bc in bcs for v in bc.values]
+    if any(len(bc.dofs) != len(bc.values) for bc in bcs):
+        raise FenicsError(FH_ERR_BAD_INPUT, "DirichletBC dofs and values differ in length")
+    d, v = _u64(dofs), _f64(vals)
+    errors = []
+    if heat_source is None:
+        cb = C.cast(None, HeatSourceFn)
+    else:
+        def _src(t, pt, _ctx):
+            try:
+                return float(heat_source(t, (pt[0], pt[1])))
+            except BaseException as e:  # re-raised after the call
+                errors.append(e)
+                return 0.0
+        cb = HeatSourceFn(_src)
+    hist = np.empty((num_steps + 1, n)) if keep_history else None
+    final = np.empty(n)
+    st = TransientStats()
+    code = lib().fh_solve_transient_heat_2d(mesh.h, float(diffusivity), _p(u0), len(d), _p(d),
+                                            _p(v), float(dt), int(num_steps), int(scheme), cb,
+                                            None, tol, max_iter, precision, _p(hist), _p(final),
+                                            C.byref(st))
+    if errors:
+        raise errors[0]
+    _check(code)
+    return TransientHeatSolution([k * dt for k in range(num_steps + 1)],
+                                 list(hist) if keep_history else [], final, st.as_dict())
 
 
 def clamp_dofs_x0(vertices, tol=1e-6):

@@ -275,6 +275,57 @@ int fh_vibration_modes(const fh_matrix* K, const fh_matrix* M, uint64_t nfix,
                        double* freq_hz, double* omega, double* eigenvalue, double* shapes,
                        uint64_t* inner_iterations);
 
+/* ------------- transient heat (the CG's consumer in transient.rs / heat_solver.rs) ------------- */
+/* replaces TimeSteppingScheme (transient.rs:12-18) */
+#define FH_SCHEME_BACKWARD_EULER 0
+#define FH_SCHEME_CRANK_NICOLSON 1
+
+/* replaces the heat_source closure FnMut(f64, &[f64; 2]) -> f64 (heat_solver.rs:32-34):
+   called with (t, vertex coordinates, ctx) */
+typedef double (*fh_heat_source_fn)(double t, const double* point, void* ctx);
+
+typedef struct fh_transient_stats {
+    uint64_t steps;
+    uint64_t cg_iterations;     /* over all steps                               */
+    uint64_t cg_iterations_low; /* of which on FP32 matrix values (FH_PREC_MIXED) */
+    uint64_t cg_iterations_max; /* largest count of one step                     */
+    double solve_ms;            /* wall time inside the per-step CG solves       */
+    double wall_ms;             /* whole call, assembly included                 */
+    int device;
+    int precision;
+} fh_transient_stats;
+
+/* replaces faer's SparseColMat at this boundary: CSC raw parts (column
+   pointers, row indices, values; gpu.rs:1552-1583 reads the same parts) ->
+   device operator, for the entries below that take fh_matrix operands */
+int fh_matrix_from_csc(uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
+                       const double* vals, int device, fh_matrix** out);
+/* replaces add_scaled_matrices (transient.rs:72-97): a + alpha b on the union
+   pattern; operands on one pattern combine on the device, others through the
+   host union of their CSC parts (a's entry, then alpha times b's, summed) */
+int fh_add_scaled_matrices(const fh_matrix* a, const fh_matrix* b, double alpha,
+                           fh_matrix** out);
+/* replaces TransientHeatSolver::step (transient.rs:32-68): BE solves
+   (M + dt K) u_next = M u_n + dt f_next, CN (M + dt/2 K) u_next =
+   (M - dt/2 K) u_n + dt/2 (f_n + f_next).  solve_direct (direct.rs:26-53, a
+   dense LU) is replaced by the device CG from x0 = u_n to ||r||/||b|| < tol,
+   so M + theta dt K must be symmetric positive definite; a CG failure is
+   returned (FH_ERR_NOT_CONVERGED / FH_ERR_BREAKDOWN).  Host vectors of n. */
+int fh_transient_heat_step(const fh_matrix* M, const fh_matrix* K, int scheme, double dt,
+                           const double* u_n, const double* f_n, const double* f_next,
+                           double tol, uint64_t max_iter, int precision, double* u_next,
+                           fh_solve_stats* stats);
+/* replaces solve_transient_heat_2d (heat_solver.rs:23-108) on a triangle
+   mesh: P1 mass and diffusivity-scaled stiffness, the Dirichlet list (the
+   DirichletBC entries concatenated in order) written into u before and after
+   every step, source sampled at the vertices at t_n and t_{n+1} (NULL: zero).
+   history [(num_steps+1)][nv] may be NULL; final_out [nv]. */
+int fh_solve_transient_heat_2d(fh_mesh* m, double diffusivity, const double* u_initial,
+                               uint64_t nbc, const uint64_t* bc_dofs, const double* bc_vals,
+                               double dt, uint64_t num_steps, int scheme, fh_heat_source_fn source,
+                               void* source_ctx, double tol, uint64_t max_iter, int precision,
+                               double* history, double* final_out, fh_transient_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

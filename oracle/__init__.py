@@ -437,3 +437,94 @@ def clamp_dofs_x0(m: Mesh, tol=1e-6):
     """elasticity_tests.rs:94-101: all 3 dofs of vertices with x < 1e-6."""
     idx = np.nonzero(m.vertices[:, 0] < tol)[0].astype(np.uint64)
     return np.sort(np.concatenate([3 * idx, 3 * idx + 1, 3 * idx + 2])).astype(np.uint64)
+
+
+# ------------------------------------------------------- transient heat
+BACKWARD_EULER, CRANK_NICOLSON = 0, 1
+
+
+def add_scaled_matrices(a: Csc, b: Csc, alpha) -> Csc:
+    """transient.rs:72-97: column by column, the triplets of a and then those
+    of alpha * b; try_new_from_triplets sums the duplicates in that order."""
+    ca, ra, va = a.export()
+    cb, rb, vb = b.export()
+    n = a.n
+    cols = np.concatenate([np.repeat(np.arange(n, dtype=np.uint64), np.diff(ca).astype(np.int64)),
+                           np.repeat(np.arange(n, dtype=np.uint64), np.diff(cb).astype(np.int64))])
+    rows = np.concatenate([ra, rb])
+    vals = np.concatenate([va, alpha * vb])
+    order = np.argsort(cols, kind="stable")  # a's entries of a column precede b's
+    return csc_from_triplets(n, rows[order], cols[order], vals[order])
+
+
+class DenseLU:
+    """direct.rs:26-53 solve_direct: the sparse matrix densified and factored
+    by partial-pivot LU (LAPACK getrf / getrs).  The reference refactors the
+    unchanged step matrix every step; factoring it once gives the same
+    factors."""
+
+    def __init__(self, A: Csc):
+        import scipy.linalg as sl
+        self.lu = sl.lu_factor(A.to_scipy().toarray(), check_finite=False)
+
+    def solve(self, b):
+        import scipy.linalg as sl
+        return sl.lu_solve(self.lu, b, check_finite=False)
+
+
+class TransientStepper:
+    """transient.rs:20-69 TransientHeatSolver for fixed (M, K): BE solves
+    (M + dt K) u' = M u + dt f', CN (M + dt/2 K) u' = (M - dt/2 K) u + dt/2 (f + f')."""
+
+    def __init__(self, M: Csc, K: Csc, scheme, dt):
+        self.cn = scheme == CRANK_NICOLSON
+        self.h = 0.5 * dt if self.cn else dt
+        self.lhs = add_scaled_matrices(M, K, self.h)
+        self.rmat = add_scaled_matrices(M, K, -self.h) if self.cn else M
+        self.lu = DenseLU(self.lhs)
+
+    def rhs(self, u_n, f_n, f_next):
+        y = self.rmat.spmv(_f64(u_n))
+        if self.cn:
+            return y + self.h * (_f64(f_n) + _f64(f_next))
+        return y + self.h * _f64(f_next)
+
+    def step(self, u_n, f_n, f_next):
+        return self.lu.solve(self.rhs(u_n, f_n, f_next))
+
+
+def transient_step(M: Csc, K: Csc, scheme, dt, u_n, f_n, f_next):
+    """transient.rs:32-68."""
+    return TransientStepper(M, K, scheme, dt).step(u_n, f_n, f_next)
+
+
+def transient_heat_2d(m: Mesh, diffusivity, u_initial, bcs, dt, num_steps, scheme,
+                      heat_source=None):
+    """heat_solver.rs:23-108.  bcs: sequence of (dofs, values); heat_source(t,
+    (x, y)) or None (zero).  Returns (time_steps, history [(steps+1), nv])."""
+    n = m.nv
+    M = assemble_mass_2d(m)
+    cp, ri, va = assemble_stiffness(m).export()
+    K = csc_from_parts(cp, ri, diffusivity * va)  # heat_solver.rs:49-58
+    stepper = TransientStepper(M, K, scheme, dt)
+    u = _f64(u_initial).copy()
+    hist = [u.copy()]
+    times = [0.0]
+    fc, fn = np.zeros(n), np.zeros(n)
+    for step in range(1, num_steps + 1):
+        t_curr, t_next = (step - 1) * dt, step * dt
+        if heat_source is not None:
+            for i in range(n):
+                pt = (m.vertices[i, 0], m.vertices[i, 1])
+                fc[i] = heat_source(t_curr, pt)
+                fn[i] = heat_source(t_next, pt)
+        for dofs, vals in bcs:
+            for d, v in zip(dofs, vals):
+                u[d] = v
+        u = stepper.step(u, fc, fn)
+        for dofs, vals in bcs:
+            for d, v in zip(dofs, vals):
+                u[d] = v
+        times.append(t_next)
+        hist.append(u.copy())
+    return times, np.array(hist)

@@ -41,6 +41,7 @@ def same_csc(a, b):
 def test_elasticity_mass_and_folds_bitwise():
     gm, om, K, M, Ko, Mo, fixed = beam(4, 2, 2, rho=7.5)
     assert same_csc(M, Mo)
+    assert M.nnz() == len(Mo.export()[2])  # the mass couples equal components only
     assert same_csc(fh.fold_dirichlet(K, fixed, True), O.fold_dirichlet(Ko, fixed, True))
     assert same_csc(fh.fold_dirichlet(M, fixed, False), O.fold_dirichlet(Mo, fixed, False))
     # duplicates and out-of-range indices (eigen.rs:57-62) are harmless
