@@ -91,6 +91,7 @@ struct CgState {
     uint32_t want_low;  // relaxation threshold crossed: switch to the FP32 operator
     uint32_t stalled;   // FP32-operator cycle failed to reduce the true residual
     double rz[2];
+    double beta[2];     // beta of iteration j, by parity (the update rebuilds p with it)
     uint32_t low[2];    // operator precision of iteration j, by parity
     double bnorm;
     double res;
@@ -276,27 +277,24 @@ __device__ __forceinline__ void row_residual_dd(const CgMat& A, uint32_t s, uint
     }
 }
 
-// z = D^{-1} r for one block row (lane-interleaved vectors); returns r.z
+// zv = D^{-1} r for one block row (lane-interleaved D); returns r.z
 template <int BS, int PRE>
 __device__ __forceinline__ double precond_row(const double* __restrict__ dinv, uint32_t s,
-                                              uint32_t lane, const double* rv, double* __restrict__ z) {
+                                              uint32_t lane, const double* rv, double* zv) {
     double rz = 0.0;
     if constexpr (PRE == 0) {
 #pragma unroll
         for (int a = 0; a < BS; ++a) {
-            const uint64_t i = ((uint64_t)s * BS + a) * SLICE + lane;
-            const double zv = rv[a] * dinv[i];
-            z[i] = zv;
-            rz += rv[a] * zv;
+            zv[a] = rv[a] * dinv[((uint64_t)s * BS + a) * SLICE + lane];
+            rz += rv[a] * zv[a];
         }
     } else {  // 3x3 block Jacobi, D lane-interleaved [slice][9][64], row-major blocks
         const double* D = dinv + (uint64_t)s * 9 * SLICE + lane;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const double zv = D[(3 * a) * SLICE] * rv[0] + D[(3 * a + 1) * SLICE] * rv[1] +
-                              D[(3 * a + 2) * SLICE] * rv[2];
-            z[((uint64_t)s * 3 + a) * SLICE + lane] = zv;
-            rz += rv[a] * zv;
+            zv[a] = D[(3 * a) * SLICE] * rv[0] + D[(3 * a + 1) * SLICE] * rv[1] +
+                    D[(3 * a + 2) * SLICE] * rv[2];
+            rz += rv[a] * zv[a];
         }
     }
     return rz;
@@ -305,13 +303,14 @@ __device__ __forceinline__ double precond_row(const double* __restrict__ dinv, u
 // t = b - A x; r = t, z = D^{-1} r; partials (t.t, r.z) and optionally b.b.
 // EXACT: t in double-double (true-residual modes: initial residual and
 // replacement) plus the rounding-floor partials; otherwise FP64 in the
-// reference's product order (ORD).  Every lane writes r and z (zeros past the
-// last row), so the padded vector entries stay zero.
+// reference's product order (ORD).  z goes to the z slots of the (z, p) pair
+// vector, p is left as it is.  Every lane writes r and z (zeros past the last
+// row), so the padded vector entries stay zero.
 template <int BS, bool ORD, bool EXACT, int PRE>
 __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restrict__ b,
                                                   const double* __restrict__ x,
                                                   const double* __restrict__ dinv,
-                                                  double* __restrict__ r, double* __restrict__ z,
+                                                  double* __restrict__ r, double* __restrict__ zp,
                                                   double2* __restrict__ red,
                                                   double* __restrict__ bb,
                                                   double* __restrict__ fl) {
@@ -347,7 +346,10 @@ __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restr
             tt += rv[a] * rv[a];
             bsq += bi[a] * bi[a];
         }
-        rz = precond_row<BS, PRE>(dinv, s, lane, rv, z);
+        double zv[BS];
+        rz = precond_row<BS, PRE>(dinv, s, lane, rv, zv);
+#pragma unroll
+        for (int a = 0; a < BS; ++a) zp[2 * (((uint64_t)s * BS + a) * SLICE + lane)] = zv[a];
     }
     tt = fhd::block_sum256(tt, sh);
     rz = fhd::block_sum256(rz, sh);
@@ -401,22 +403,30 @@ __device__ __forceinline__ void ld_pair(const T* p, T& a, T& b) {
 // loaded before this step's gathers.  No per-lane length test: padded SELL
 // entries hold zero values and the row's own column (or column 0 past the
 // last row), so they add exact zeros.
-template <int BS, int U, class TM, bool DIFF, bool NT, class G>
+// With NP = 2 the slice's steps are split in two halves (part 0: the first,
+// rounded up to whole steps of U) for two waves to run side by side.
+template <int BS, int U, class TM, bool DIFF, bool NT, int NP = 1, class G>
 __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __restrict__ vals,
                                                  uint32_t s, uint32_t lane, const double* xr,
-                                                 G&& gather, double* acc) {
+                                                 G&& gather, double* acc, uint32_t part = 0) {
     constexpr int B2 = BS * BS;
     const uint32_t base = A.soff[s], w = (A.soff[s + 1] - base) / SLICE;
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    if (w == 0) return;
+    uint32_t k0 = 0, k1 = w;
+    if constexpr (NP == 2) {
+        const uint32_t mid = min(w, ((w + 1) / 2 + U - 1) / U * U);
+        k0 = part ? mid : 0;
+        k1 = part ? w : mid;
+    }
+    if (k1 <= k0) return;
     const uint32_t* cp = A.cols + (uint64_t)base + lane;
     const TM* vb = vals + (uint64_t)base * B2;  // step k: vb + k * SLICE * B2, see vpos
-    const uint32_t last = w - 1;
+    const uint32_t last = k1 - 1;
     uint32_t c[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min((uint32_t)u, last) * SLICE);
-    for (uint32_t k = 0; k < w; k += U) {
+    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min(k0 + u, last) * SLICE);
+    for (uint32_t k = k0; k < k1; k += U) {
         uint32_t cn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -439,8 +449,8 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             gather(c[u], xg[u]);
-            // steps past the slice width re-read its last entry: add zero
-            const bool ok = u == 0 || k + u < w;
+            // steps past the range re-read its last entry: add zero
+            const bool ok = u == 0 || k + u < k1;
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
                 const double d = DIFF ? xg[u][q] - xr[q] : xg[u][q];
@@ -466,15 +476,25 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 // gathered, A p_old is the previous Ap of the lane's own row.  The recurrence
 // drifts from the exact product by rounding, so it is used only in the
 // true-residual modes, whose replacements follow an exact product.
-template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false>
-__global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ z, double* p0,
-                                                 double* p1, double* __restrict__ Ap, CgState* st,
+//
+// zp holds (z_i, p_i) pairs at lane-interleaved index i (one 16-byte gather per
+// component).  The kernel forms p_new = z + beta p_old on the fly for every
+// gathered column and for its own rows; it writes only Ap: the update kernel
+// rebuilds the same p_new (same beta, same arithmetic) and stores the pairs.
+// SP = 2: two waves per slice, each over half of its stored blocks, combined
+// through LDS (a block holds two slices): twice the waves in flight for the
+// same rows.
+template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false, int SP = 1>
+__global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ zp,
+                                                 double* __restrict__ Ap, CgState* st,
                                                  const double2* __restrict__ red_in, int n_in,
                                                  const double* __restrict__ bb_in,
                                                  const double* __restrict__ fl_in,
                                                  double* __restrict__ red_out) {
     static_assert(DIFF || !LOW, "the FP32 operator is only used in difference form");
+    static_assert(SP == 1 || !ORD, "the ordered product runs one wave per slice");
     __shared__ double sh[4];
+    __shared__ double half_acc[SP == 2 ? 2 * BS * SLICE : 1];
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
@@ -603,35 +623,38 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
     }
     if (lead) {
         st->rz[j & 1] = rz;
+        st->beta[j & 1] = beta;
         st->low[j & 1] = LOW ? 1u : 0u;
         st->it_spmv = j;
         if (LOW && !low_prev) st->it_low = j;
         if (!LOW && mode == M_MIXED && rel < st->sw) st->want_low = 1;
     }
-    const double* pold = (j & 1) ? p0 : p1;
-    double* pnew = (j & 1) ? p1 : p0;
 
     const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
     // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
-    const unsigned s = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+    const unsigned wv = threadIdx.x >> 6;
+    const unsigned s = __builtin_amdgcn_readfirstlane(SP == 2 ? blk * 2 + (wv >> 1) : blk * 4 + wv);
+    const unsigned part = SP == 2 ? (wv & 1) : 0;
     const unsigned lane = threadIdx.x & 63;
     double dot = 0.0;
+    double acc[BS], xr[BS];
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        double xr[BS], xz[BS], acc[BS];
+        double xz[BS];
 #pragma unroll
         for (int q = 0; q < BS; ++q) {
             const uint64_t i = ((uint64_t)s * BS + q) * SLICE + lane;
-            xz[q] = z[i];
-            xr[q] = madd(xz[q], beta, pold[i], ORD);
+            double po;
+            ld_pair<false>(zp + 2 * i, xz[q], po);
+            xr[q] = madd(xz[q], beta, po, ORD);
         }
         auto gather = [&](uint32_t c, double* xv) {
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
-                const uint64_t ci = li<BS>(c, q);
-                if constexpr (REC) xv[q] = z[ci];
-                else xv[q] = madd(z[ci], beta, pold[ci], ORD);
+                double zc, pc;
+                ld_pair<false>(zp + 2 * li<BS>(c, q), zc, pc);
+                xv[q] = REC ? zc : madd(zc, beta, pc, ORD);
             }
         };
         // own-row entry of the gathered vector (difference form)
@@ -640,20 +663,34 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
         // measured 2-4% faster than two), two for FP32 ones, four scalars
         constexpr int U = BS == 1 ? 4 : (LOW ? 2 : 1);
         if constexpr (LOW) {
-            row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
-            add_rowsum<BS>(A.S32, s, lane, xo, acc);
+            row_product_pipe<BS, U, float, true, false, SP>(A, A.v32, s, lane, xo, gather, acc,
+                                                            part);
+            if (part == 0) add_rowsum<BS>(A.S32, s, lane, xo, acc);
         } else if constexpr (ORD) {
             row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
         } else {
-            row_product_pipe<BS, U, double, DIFF, true>(A, A.v64, s, lane, xo, gather, acc);
-            if (DIFF) add_rowsum<BS>(A.S64, s, lane, xo, acc);
+            row_product_pipe<BS, U, double, DIFF, true, SP>(A, A.v64, s, lane, xo, gather, acc,
+                                                            part);
+            if (DIFF && part == 0) add_rowsum<BS>(A.S64, s, lane, xo, acc);
         }
+    }
+    if constexpr (SP == 2) {
+        // the second half hands its sums to the first
+        double* h = half_acc + (wv >> 1) * BS * SLICE + lane;
+        if (part == 1 && s < A.nslices)
+#pragma unroll
+            for (int a = 0; a < BS; ++a) h[a * SLICE] = acc[a];
+        __syncthreads();
+        if (part == 0 && s < A.nslices)
+#pragma unroll
+            for (int a = 0; a < BS; ++a) acc[a] += h[a * SLICE];
+    }
+    if (part == 0 && s < A.nslices) {
         // every lane stores (rows past the last hold zeros): full 512-byte stores
 #pragma unroll
         for (int a = 0; a < BS; ++a) {
             const uint64_t i = ((uint64_t)s * BS + a) * SLICE + lane;
             if constexpr (REC) acc[a] = fma(beta, Ap[i], acc[a]);
-            pnew[i] = xr[a];
             Ap[i] = acc[a];
             dot += xr[a] * acc[a];
         }
@@ -664,14 +701,17 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
 
 // PRE 0: scalar Jacobi (cg.rs:52-64), elementwise over the padded vectors;
 // PRE 1: 3x3 block Jacobi (gpu.rs bsr3_apply_dinv), one thread per block row.
+// p_new = z + beta p_old is rebuilt from the (z, p) pairs exactly as the spmv
+// kernel formed it; the pairs are overwritten with (z_new, p_new).
 template <bool ORD, int PRE>
 __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __restrict__ x,
                                                    double* __restrict__ r,
-                                                   const double* __restrict__ Ap, const double* p0,
-                                                   const double* p1, const double* __restrict__ dinv,
-                                                   double* __restrict__ z, CgState* st,
+                                                   const double* __restrict__ Ap,
+                                                   double* __restrict__ zp,
+                                                   const double* __restrict__ dinv, CgState* st,
                                                    const double* __restrict__ red_in, int n_in,
                                                    double2* __restrict__ red_out) {
+    typedef double pair_t __attribute__((ext_vector_type(2)));
     __shared__ double sh[4];
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_spmv;
@@ -686,18 +726,24 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
         return;
     }
     const double al = st->rz[j & 1] / pap;
-    const double* p = (j & 1) ? p1 : p0;
+    const double beta = st->beta[j & 1];
+    pair_t* zpv = reinterpret_cast<pair_t*>(zp);
     double rr = 0.0, rz = 0.0;
     const uint32_t stride = gridDim.x * 256;
     if (PRE == 0) {
         for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < npad; i += stride) {
-            // cg.rs:113-116, 125-127
-            const double xi = ORD ? x[i] + al * p[i] : fma(al, p[i], x[i]);
+            // cg.rs:112-115 with p of cg.rs:133-135, z of cg.rs:124-126
+            const pair_t o = zpv[i];
+            const double pi = madd(o.x, beta, o.y, ORD);
+            const double xi = ORD ? x[i] + al * pi : fma(al, pi, x[i]);
             const double ri = ORD ? r[i] - al * Ap[i] : fma(-al, Ap[i], r[i]);
             const double zi = ri * dinv[i];
             x[i] = xi;
             r[i] = ri;
-            z[i] = zi;
+            pair_t o2;
+            o2.x = zi;
+            o2.y = pi;
+            zpv[i] = o2;
             rr += ri * ri;
             rz += ri * zi;
         }
@@ -705,16 +751,25 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
         const uint32_t nn = npad / 3;
         for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < nn; v += stride) {
             const uint32_t s = v / SLICE, lane = v % SLICE;
-            double rv[3];
+            double rv[3], pv[3], zv[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const uint64_t i = ((uint64_t)s * 3 + a) * SLICE + lane;
-                x[i] = ORD ? x[i] + al * p[i] : fma(al, p[i], x[i]);
+                const pair_t o = zpv[i];
+                pv[a] = madd(o.x, beta, o.y, ORD);
+                x[i] = ORD ? x[i] + al * pv[a] : fma(al, pv[a], x[i]);
                 rv[a] = ORD ? r[i] - al * Ap[i] : fma(-al, Ap[i], r[i]);
                 r[i] = rv[a];
                 rr += rv[a] * rv[a];
             }
-            rz += precond_row<3, 1>(dinv, s, lane, rv, z);
+            rz += precond_row<3, 1>(dinv, s, lane, rv, zv);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                pair_t o2;
+                o2.x = zv[a];
+                o2.y = pv[a];
+                zpv[((uint64_t)s * 3 + a) * SLICE + lane] = o2;
+            }
         }
     }
     rr = fhd::block_sum256(rr, sh);
@@ -952,7 +1007,8 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     if (uses32 || diff64) ensure_rowsum(am, uses32);
     const CgMat A = cgmat(a);
 
-    DBuf<double> r(dev, npad), z(dev, npad), p0(dev, npad), p1(dev, npad), Ap(dev, npad);
+    // zp: (z, p) pairs at each lane-interleaved index
+    DBuf<double> r(dev, npad), zp(dev, 2 * (size_t)npad), Ap(dev, npad);
     DBuf<double> xl(dev, npad), bl(dev, npad);
     // true-residual modes: the iterate at each of the last two replacements
     DBuf<double> xsave[2];
@@ -971,10 +1027,9 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     to_li<BS>(args.b, bl.p, nrows, nsl, 0.0, s);
     if (args.x0) to_li<BS>(args.x0, xl.p, nrows, nsl, 0.0, s);
     else fill_f64(xl.p, npad, 0.0, s);
-    fill_f64(p0.p, npad, 0.0, s);
-    fill_f64(p1.p, npad, 0.0, s);
+    fill_f64(zp.p, 2 * (size_t)npad, 0.0, s);
     double* x = xl.p;
-    DBuf<double> redA(dev, gs), bb(dev, gs), fl(dev, gs);
+    DBuf<double> redA(dev, std::max(gs, (unsigned)cdiv(nsl, 2))), bb(dev, gs), fl(dev, gs);
     DBuf<double2> redB(dev, std::max(gs, gu));
 
     CgState* hs = host_state(dev);
@@ -994,7 +1049,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // r0 = b - A x0 (FP64; double-double in the true-residual modes),
     // z0 = D^{-1} r0, partials (r.r, r.z), b.b
     hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, bl.p, x,
-                       dinv.p, r.p, z.p, redB.p, bb.p, fl.p);
+                       dinv.p, r.p, zp.p, redB.p, bb.p, fl.p);
     FH_LAUNCH_CHECK("residual");
     int n_in_spmv = (int)gs;  // partial count feeding the next spmv kernel
 
@@ -1013,32 +1068,41 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
     // the initial residual or a replacement, where the exact product restarts it
     const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
+    // FH_CG_SPLIT=2: two waves per slice in the CG product (not in ordered mode)
+    const bool split = !ORD && env_int("FH_CG_SPLIT", 1) == 2;
+    const unsigned gsp = split ? std::max(1u, cdiv(nsl, 2)) : gs;  // CG spmv grid
     auto launch_chunk = [&](int nin_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m], s));
             const int nin = m == 0 ? nin_first : (int)gu;
             const bool rec = aprec && !(m == 0 && nin_first == (int)gs);
-#define FH_SPMV(D, L)                                                                          \
-    do {                                                                                       \
-        if (rec)                                                                               \
-            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, true>), dim3(gs), dim3(256), 0, s, A, \
-                               z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);  \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, false>), dim3(gs), dim3(256), 0, s, A, \
-                               z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);  \
+#define FH_SPMV_SP(D, L, R, SPV)                                                                 \
+    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R, SPV>), dim3(gsp), dim3(256), 0, s, A, zp.p,  \
+                       Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p)
+#define FH_SPMV(D, L, R)                                      \
+    do {                                                      \
+        if constexpr (ORD) FH_SPMV_SP(D, L, R, 1);            \
+        else if (split) FH_SPMV_SP(D, L, R, 2);               \
+        else FH_SPMV_SP(D, L, R, 1);                          \
     } while (0)
             if constexpr (DIFF) {
-                if (lowk) FH_SPMV(true, true);
-                else if (diff64) FH_SPMV(true, false);
-                else FH_SPMV(false, false);
+                if (rec) {
+                    if (lowk) FH_SPMV(true, true, true);
+                    else if (diff64) FH_SPMV(true, false, true);
+                    else FH_SPMV(false, false, true);
+                } else {
+                    if (lowk) FH_SPMV(true, true, false);
+                    else if (diff64) FH_SPMV(true, false, false);
+                    else FH_SPMV(false, false, false);
+                }
             } else {
-                hipLaunchKernelGGL((k_cg_spmv<BS, ORD, false, false>), dim3(gs), dim3(256), 0, s, A,
-                                   z.p, p0.p, p1.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);
+                FH_SPMV(false, false, false);
             }
 #undef FH_SPMV
+#undef FH_SPMV_SP
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m + 1], s));
             hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, npad, x, r.p,
-                               Ap.p, p0.p, p1.p, dinv.p, z.p, st.p, redA.p, (int)gs, redB.p);
+                               Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gsp, redB.p);
         }
     };
     uint64_t launched = 0;
@@ -1096,7 +1160,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             d2d(xsave[hs->n_rep & 1].p, x, npad, s);
             // residual replacement: r = b - A x (double-double evaluation)
             hipLaunchKernelGGL((k_residual<BS, false, true, PRE>), dim3(gs), dim3(256), 0, s, A,
-                               bl.p, x, dinv.p, r.p, z.p, redB.p, (double*)nullptr, fl.p);
+                               bl.p, x, dinv.p, r.p, zp.p, redB.p, (double*)nullptr, fl.p);
             hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, st.p);
             FH_LAUNCH_CHECK("replacement");
             n_in_spmv = (int)gs;
@@ -1148,10 +1212,10 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         stats->spmv_bytes = diff64 ? diff_spmv_bytes(a, 8) : spmv_algorithmic_bytes(a, 8, 8);
         stats->spmv32_bytes = diff_spmv_bytes(a, 4);
         // SpMV + the vector passes of the fused loop: the spmv kernel gathers
-        // z and p_old and writes p_new and Ap, the update reads x, r, Ap, p,
-        // D^{-1} and writes x, r, z
+        // (z, p_old) pairs (one vector beyond the SpMV's x) and writes Ap, the
+        // update reads x, r, Ap, the pairs and D^{-1} and writes x, r, the pairs
         const double dinv_b = PRE == 0 ? 8.0 * n : 24.0 * n;
-        stats->iter_bytes = stats->spmv_bytes + 8.0 * n * 10.0 + dinv_b - 8.0 * n * 2.0;
+        stats->iter_bytes = stats->spmv_bytes + 8.0 * n * (1.0 + 9.0) + dinv_b;
         const double it = (double)std::max<uint64_t>(stats->iterations, 1);
         stats->effective_bw_gbps = stats->iter_bytes * it / (kms * 1e-3) / 1e9;
         const double flops = 2.0 * (double)a->nnz_scalar + 12.0 * n;
