@@ -185,6 +185,42 @@ __global__ __launch_bounds__(256) void k_asm_elasticity_mass(
     }
 }
 
+// mass.rs:136-180: the P2 vector mass, M_{(i,al),(j,al)} += density |det J|
+// I_ab per cell (I_ab the reference-tetrahedron product of P2 basis
+// functions a and b), the same scalar on each component.  Nodes [n][3],
+// cells [nc][10] (4 corners, then edges 01, 12, 02, 03, 13, 23); the map
+// uses the corners only.
+struct P2MassTab {
+    double m[100];
+};
+__global__ __launch_bounds__(256) void k_asm_p2_vec_mass(
+    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nd,
+    double density, P2MassTab tab, double* __restrict__ vals) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    const RowCtx rc = row_ctx(soff, rlen, d);
+    init_row<9>(vals, rc);
+    for (uint32_t e = aptr[d]; e < aptr[d + 1]; ++e) {
+        const uint32_t c = aent[e] / 10, a = aent[e] % 10;
+        double x[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) x[k][q] = V[3 * (uint64_t)C[(uint64_t)c * 10 + k] + q];
+        Map3 m;
+        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        const double det = fabs(m.det);
+        for (int b = 0; b < 10; ++b) {
+            const uint32_t k = slots[(uint64_t)e * 10 + b];
+            const double mv = density * det * tab.m[a * 10 + b];
+#pragma unroll
+            for (int al = 0; al < 3; ++al) vals[vidx(rc.soff, k, al * 3 + al, rc.lane, 9)] += mv;
+        }
+    }
+}
+
 // Dirichlet fold (eigen.rs:51-78): the diagonal value of a constrained dof is
 // `diag` (1 for the stiffness, 0 for the mass), whatever its multiplicity.
 __global__ void k_fold_count(const uint32_t* __restrict__ count, uint32_t n, uint32_t diag,
@@ -543,6 +579,80 @@ fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density) {
     return a.release();
 }
 
+// mass.rs:190-250: int phi_a phi_b over the reference tetrahedron, from the
+// barycentric moments prod m_i! / (sum m_i + 3)!; corner node i is
+// lambda_i (2 lambda_i - 1), edge node (i, j) is 4 lambda_i lambda_j.  Host
+// arithmetic in the reference's term order.
+static P2MassTab p2_mass_table() {
+    static const double F[12] = {1.0,   1.0,    2.0,     6.0,      24.0,      120.0,
+                                 720.0, 5040.0, 40320.0, 362880.0, 3628800.0, 39916800.0};
+    struct Term {
+        int e[4];
+        double c;
+    };
+    std::vector<std::vector<Term>> T(10);
+    for (int i = 0; i < 4; ++i) {
+        Term sq{{0, 0, 0, 0}, 2.0}, lin{{0, 0, 0, 0}, -1.0};
+        sq.e[i] = 2;
+        lin.e[i] = 1;
+        T[i] = {sq, lin};
+    }
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    for (int k = 0; k < 6; ++k) {
+        Term t{{0, 0, 0, 0}, 4.0};
+        t.e[E[k][0]] = 1;
+        t.e[E[k][1]] = 1;
+        T[4 + k] = {t};
+    }
+    P2MassTab tab{};
+    for (int a = 0; a < 10; ++a)
+        for (int b = 0; b < 10; ++b) {
+            double acc = 0.0;
+            for (const Term& ta : T[a])
+                for (const Term& tb : T[b]) {
+                    int m[4], deg = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        m[q] = ta.e[q] + tb.e[q];
+                        deg += m[q];
+                    }
+                    acc += ta.c * tb.c * F[m[0]] * F[m[1]] * F[m[2]] * F[m[3]] / F[deg + 3];
+                }
+            tab.m[a * 10 + b] = acc;
+        }
+    return tab;
+}
+
+fh_matrix* assemble_elasticity_mass_p2(int dev, uint64_t n_nodes, const double* nodes,
+                                       uint64_t nc, const uint64_t* cells, double density) {
+    FH_CHECK(n_nodes < 0x3fffffffull && nc * 10 < 0xffffffffull, "mesh too large");
+    FH_CHECK(nc == 0 || (nodes && cells), "null mesh arrays");
+    std::vector<uint32_t> c32(nc * 10);
+    for (uint64_t i = 0; i < nc * 10; ++i) {
+        FH_CHECK(cells[i] < n_nodes, "cell node index out of range");
+        c32[i] = (uint32_t)cells[i];
+    }
+    DeviceCtx& d = device(dev);
+    hipStream_t s = d.stream;
+    DBuf<double> V(dev, std::max<uint64_t>(3 * n_nodes, 1));
+    DBuf<uint32_t> C(dev, std::max<uint64_t>(nc * 10, 1));
+    h2d(V.p, nodes, 3 * n_nodes, s);
+    h2d(C.p, c32.data(), nc * 10, s);
+    auto adj = build_adjacency(dev, C.p, (uint32_t)nc, 10, (uint32_t)n_nodes, s);
+    auto pat = build_pattern(dev, C.p, 10, *adj, s);
+    build_slots(C.p, 10, *adj, *pat, s);
+    std::unique_ptr<fh_matrix> a(new_matrix(dev, pat, 3));
+    if (n_nodes)
+        hipLaunchKernelGGL(k_asm_p2_vec_mass, dim3(cdiv(n_nodes, 256)), dim3(256), 0, s, V.p, C.p,
+                           adj->ptr.p, adj->ent.p, adj->slots.p, pat->slice_off.p,
+                           pat->row_len.p, (uint32_t)n_nodes, density, p2_mass_table(),
+                           a->vals.p);
+    FH_LAUNCH_CHECK("asm_p2_vec_mass");
+    FH_HIP(hipStreamSynchronize(s));
+    a->comp_diag = true;
+    a->nnz_scalar = pat->nnzb * 3;
+    return a.release();
+}
+
 // Condense a copy of `a` (values shared pattern) and lift b (device).
 fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
                            const uint64_t* dofs, const double* vals, double* b_out,
@@ -718,6 +828,16 @@ int fh_assemble_mass(fh_mesh* m, fh_matrix** out) {
         FH_CHECK(m && out, "null argument");
         *out = assemble_p1(m, 1);
         FH_HIP(hipStreamSynchronize(stream_of(m->dev)));
+    });
+}
+
+int fh_assemble_elasticity_mass_p2(uint64_t n_nodes, const double* nodes, uint64_t nc,
+                                   const uint64_t* cells, double density, int device_idx,
+                                   fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(out, "null argument");
+        *out = assemble_elasticity_mass_p2(resolve_device(device_idx), n_nodes, nodes, nc, cells,
+                                           density);
     });
 }
 

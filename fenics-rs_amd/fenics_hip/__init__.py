@@ -157,9 +157,15 @@ def lib():
                                                   vp, vp, C.POINTER(SolveStats)]),
         "fh_assemble_elasticity_mass": (C.c_int, [vp, dbl, PP]),
         "fh_fold_dirichlet": (C.c_int, [vp, u64, vp, C.c_int, PP]),
+        "fh_assemble_elasticity_mass_p2": (C.c_int, [u64, vp, u64, vp, dbl, C.c_int, PP]),
         "fh_vibration_modes": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl, dbl, u64, C.c_int,
                                          vp, vp, vp, vp, C.POINTER(u64)]),
         "fh_matrix_from_csc": (C.c_int, [u64, vp, vp, vp, C.c_int, PP]),
+        "fh_mesh_boundary_facets": (C.c_int, [vp, vp, vp, vp, C.POINTER(u64)]),
+        "fh_boundary_points": (C.c_int, [vp, u64, vp, vp]),
+        "fh_assemble_neumann_2d": (C.c_int, [vp, u64, vp, vp, vp]),
+        "fh_assemble_surface_traction_3d": (C.c_int, [vp, u64, vp, vp, vp]),
+        "fh_assemble_robin_matrix_2d": (C.c_int, [vp, u64, vp, dbl, PP]),
         "fh_add_scaled_matrices": (C.c_int, [vp, vp, dbl, PP]),
         "fh_transient_heat_step": (C.c_int, [vp, vp, C.c_int, dbl, vp, vp, vp, dbl, u64,
                                              C.c_int, vp, C.POINTER(SolveStats)]),
@@ -785,6 +791,14 @@ def assemble_elasticity_mass_3d(mesh: Mesh, density: float) -> SparseMatrix:
     return _new_matrix(lib().fh_assemble_elasticity_mass, mesh.h, float(density))
 
 
+def assemble_elasticity_mass_p2_3d(nodes, cells, density: float, device: int = -1) -> SparseMatrix:
+    """mass.rs:136-180: P2 nodes [n, 3], cells [nc, 10] (corners, then edges
+    01, 12, 02, 03, 13, 23)."""
+    v, c = _f64(nodes).reshape(-1, 3), _u64(cells).reshape(-1, 10)
+    return _new_matrix(lib().fh_assemble_elasticity_mass_p2, len(v), _p(v), len(c), _p(c),
+                       float(density), device)
+
+
 def fold_dirichlet(mat: SparseMatrix, fixed_dofs, unit_diag: bool) -> SparseMatrix:
     """eigen.rs:51-78."""
     d = _u64(fixed_dofs)
@@ -824,6 +838,81 @@ def solve_vibration_modes(k: SparseMatrix, m: SparseMatrix, fixed_dofs, num_mode
 
 class _ModeList(list):
     inner_iterations = 0
+
+
+# ---------------------------------------------- boundary (ds) integrals
+@dataclasses.dataclass
+class BoundaryFacet:
+    """mesh.rs BoundaryFacet: vertices (2D the sorted edge, 3D outward
+    order), the cell holding it and its local facet index."""
+    vertices: tuple
+    cell_idx: int
+    local_facet_idx: int
+
+
+def boundary_facet_arrays(mesh: Mesh):
+    """extract_boundary_facets as arrays: (vertices [nf, dim], cell [nf],
+    local [nf]), sorted by the sorted vertex tuple."""
+    n = u64(0)
+    _check(lib().fh_mesh_boundary_facets(mesh.h, None, None, None, C.byref(n)))
+    nf = int(n.value)
+    dim = mesh.vertices.shape[1]
+    v = np.empty((nf, dim), np.uint64)
+    c = np.empty(nf, np.uint64)
+    l = np.empty(nf, np.uint64)  # noqa: E741
+    if nf:
+        _check(lib().fh_mesh_boundary_facets(mesh.h, _p(v), _p(c), _p(l), C.byref(n)))
+    return v, c, l
+
+
+def extract_boundary_facets(mesh: Mesh):
+    """mesh.rs:79-113 / 145-210."""
+    v, c, l = boundary_facet_arrays(mesh)
+    return [BoundaryFacet(tuple(int(a) for a in v[i]), int(c[i]), int(l[i])) for i in range(len(c))]
+
+
+def _facet_array(facets, dim):
+    if isinstance(facets, np.ndarray):
+        return _u64(facets).reshape(-1, dim)
+    return _u64([f.vertices if isinstance(f, BoundaryFacet) else f for f in facets]).reshape(-1, dim)
+
+
+def boundary_points(mesh: Mesh, facets) -> np.ndarray:
+    """Where boundary.rs samples its closures: [nf, 2, 2] Gauss points (2D),
+    [nf, 3, 3] edge midpoints (3D)."""
+    dim = mesh.vertices.shape[1]
+    f = _facet_array(facets, dim)
+    out = np.empty((len(f), 2, 2) if dim == 2 else (len(f), 3, 3))
+    _check(lib().fh_boundary_points(mesh.h, len(f), _p(f), _p(out)))
+    return out
+
+
+def assemble_neumann_2d(mesh: Mesh, boundary_facets, flux_fn: Callable) -> np.ndarray:
+    """boundary.rs:8-46: flux_fn((x, y)) -> float, sampled facet by facet."""
+    f = _facet_array(boundary_facets, 2)
+    pts = boundary_points(mesh, f)
+    g = np.array([[flux_fn((p[0], p[1])) for p in fp] for fp in pts], np.float64).reshape(len(f), 2)
+    out = np.empty(mesh.num_vertices())
+    _check(lib().fh_assemble_neumann_2d(mesh.h, len(f), _p(f), _p(g), _p(out)))
+    return out
+
+
+def assemble_surface_traction_3d(mesh: Mesh, boundary_facets, traction_fn: Callable) -> np.ndarray:
+    """boundary.rs:49-108: traction_fn((x, y, z)) -> (tx, ty, tz) at each
+    facet's edge midpoints."""
+    f = _facet_array(boundary_facets, 3)
+    mids = boundary_points(mesh, f)
+    t = np.array([[traction_fn(tuple(p)) for p in fm] for fm in mids],
+                 np.float64).reshape(len(f), 3, 3)
+    out = np.empty(3 * mesh.num_vertices())
+    _check(lib().fh_assemble_surface_traction_3d(mesh.h, len(f), _p(f), _p(t), _p(out)))
+    return out
+
+
+def assemble_robin_matrix_2d(mesh: Mesh, boundary_facets, gamma: float) -> SparseMatrix:
+    """boundary.rs:111-139."""
+    f = _facet_array(boundary_facets, 2)
+    return _new_matrix(lib().fh_assemble_robin_matrix_2d, mesh.h, len(f), _p(f), float(gamma))
 
 
 # ------------------------------------------------------- transient heat

@@ -258,6 +258,12 @@ int fh_batched_poisson_block_cg(uint64_t n_inst, const uint64_t* sizes, uint64_t
 /* replaces assemble_elasticity_mass_3d (mass.rs:86-126): 3x3-block vector mass
    M_{(i,a),(j,a)} = density * vol * (1/10 diagonal, 1/20 off) per tetrahedron */
 int fh_assemble_elasticity_mass(fh_mesh* m, double density, fh_matrix** out);
+/* replaces assemble_elasticity_mass_p2_3d (mass.rs:136-180) on a P2 mesh given
+   as node positions [n_nodes][3] and cells [nc][10] (4 corners, then edges
+   01, 12, 02, 03, 13, 23): M_{(i,a),(j,a)} = density |det J| int phi_i phi_j */
+int fh_assemble_elasticity_mass_p2(uint64_t n_nodes, const double* nodes, uint64_t nc,
+                                   const uint64_t* cells, double density, int device,
+                                   fh_matrix** out);
 /* replaces fold_dirichlet (eigen.rs:51-78): constrained rows and columns
    removed, constrained diagonal 1 (unit_diag) or 0; the CSC export drops exact
    zeros as the reference does */
@@ -274,6 +280,33 @@ int fh_vibration_modes(const fh_matrix* K, const fh_matrix* M, uint64_t nfix,
                        double tol, double inner_tol, uint64_t inner_max_iter, int precision,
                        double* freq_hz, double* omega, double* eigenvalue, double* shapes,
                        uint64_t* inner_iterations);
+
+/* ------------------------------- boundary facets and boundary (ds) integrals ------------------------------- */
+/* replaces extract_boundary_facets (mesh.rs:79-113 triangles, 145-210 tetrahedra):
+   per facet its vertices ([count][dim]; 2D the sorted edge, 3D the face with
+   an outward normal (v1 - v0) x (v2 - v0)), cell_idx and local_facet_idx, in
+   the order of the sorted vertex tuples.  Query-then-fill: with all three
+   outputs NULL only *count is set; otherwise *count is the capacity (0: do not
+   check) and receives the facet count. */
+int fh_mesh_boundary_facets(fh_mesh* m, uint64_t* vertices, uint64_t* cell_idx,
+                            uint64_t* local_idx, uint64_t* count);
+/* the points where boundary.rs samples its closures, facet by facet: on a
+   triangle mesh the 2-point Gauss points (1 - s) v0 + s v1 of each edge
+   ([nf][2][2], boundary.rs:29-36), on a tetrahedral mesh the edge midpoints
+   v0v1, v1v2, v2v0 of each face ([nf][3][3], boundary.rs:77-93) */
+int fh_boundary_points(fh_mesh* m, uint64_t nf, const uint64_t* facets, double* points);
+/* replaces assemble_neumann_2d (boundary.rs:8-46): g[nf][2] the flux at the
+   fh_boundary_points of each facet; rhs_out [nv] */
+int fh_assemble_neumann_2d(fh_mesh* m, uint64_t nf, const uint64_t* facets, const double* g,
+                           double* rhs_out);
+/* replaces assemble_surface_traction_3d (boundary.rs:49-108): t[nf][3][3] the
+   traction at the face's edge midpoints; rhs_out [3 nv] */
+int fh_assemble_surface_traction_3d(fh_mesh* m, uint64_t nf, const uint64_t* facets,
+                                    const double* t, double* rhs_out);
+/* replaces assemble_robin_matrix_2d (boundary.rs:111-139); a facet that
+   repeats a vertex is rejected (FH_ERR_BAD_INPUT) */
+int fh_assemble_robin_matrix_2d(fh_mesh* m, uint64_t nf, const uint64_t* facets, double gamma,
+                                fh_matrix** out);
 
 /* ------------- transient heat (the CG's consumer in transient.rs / heat_solver.rs) ------------- */
 /* replaces TimeSteppingScheme (transient.rs:12-18) */

@@ -105,6 +105,19 @@ def lib():
             "or_element_stresses": (C.c_int, [f64p, C.c_uint64, u64p, f64p, C.c_double,
                                               C.c_double, f64p]),
             "or_vertex_von_mises": (C.c_int, [C.c_uint64, f64p, C.c_uint64, u64p, f64p, f64p]),
+            "or_p2_mass_products": (None, [f64p]),
+            "or_assemble_elasticity_mass_p2_3d": (vp, [C.c_uint64, f64p, C.c_uint64, u64p,
+                                                       C.c_double, C.c_int]),
+            "or_boundary_facets_full": (C.c_int, [C.c_int, f64p, C.c_uint64, u64p, vp, vp, vp,
+                                                  C.POINTER(C.c_uint64)]),
+            "or_neumann_points_2d": (C.c_int, [f64p, C.c_uint64, u64p, f64p]),
+            "or_assemble_neumann_2d": (C.c_int, [C.c_uint64, f64p, C.c_uint64, u64p, f64p,
+                                                 f64p]),
+            "or_traction_points_3d": (C.c_int, [f64p, C.c_uint64, u64p, f64p]),
+            "or_assemble_surface_traction_3d": (C.c_int, [C.c_uint64, f64p, C.c_uint64, u64p,
+                                                          f64p, f64p]),
+            "or_assemble_robin_matrix_2d": (vp, [C.c_uint64, f64p, C.c_uint64, u64p,
+                                                 C.c_double]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -528,3 +541,72 @@ def transient_heat_2d(m: Mesh, diffusivity, u_initial, bcs, dt, num_steps, schem
         times.append(t_next)
         hist.append(u.copy())
     return times, np.array(hist)
+
+
+# ------------------------------------------------- boundary (ds) integrals
+def boundary_facets_full(m: Mesh):
+    """mesh.rs:79-113 / 145-210: (vertices [nf, dim] (3D: outward order),
+    cell [nf], local facet [nf]), sorted by the sorted vertex tuple."""
+    n = C.c_uint64()
+    lib().or_boundary_facets_full(m.dim, m.vertices, m.nc, m.cells, None, None, None,
+                                  C.byref(n))
+    v = np.empty((n.value, m.dim), np.uint64)
+    c = np.empty(n.value, np.uint64)
+    l = np.empty(n.value, np.uint64)  # noqa: E741
+    lib().or_boundary_facets_full(m.dim, m.vertices, m.nc, m.cells, v.ctypes.data, c.ctypes.data,
+                                  l.ctypes.data, C.byref(n))
+    return v, c, l
+
+
+def neumann_points_2d(m: Mesh, facets):
+    f = _u64(facets).reshape(-1, 2)
+    out = np.empty((len(f), 2, 2))
+    lib().or_neumann_points_2d(m.vertices, len(f), f, out.reshape(-1))
+    return out
+
+
+def assemble_neumann_2d(m: Mesh, facets, flux_fn):
+    """boundary.rs:8-46: flux_fn((x, y)) called per facet, per Gauss point."""
+    f = _u64(facets).reshape(-1, 2)
+    pts = neumann_points_2d(m, f)
+    g = np.array([[flux_fn((p[0], p[1])) for p in fp] for fp in pts], dtype=np.float64)
+    out = np.empty(m.nv)
+    lib().or_assemble_neumann_2d(m.nv, m.vertices, len(f), f, g.reshape(-1), out)
+    return out
+
+
+def traction_points_3d(m: Mesh, facets):
+    f = _u64(facets).reshape(-1, 3)
+    out = np.empty((len(f), 3, 3))
+    lib().or_traction_points_3d(m.vertices, len(f), f, out.reshape(-1))
+    return out
+
+
+def assemble_surface_traction_3d(m: Mesh, facets, traction_fn):
+    """boundary.rs:49-108: traction_fn((x, y, z)) -> (tx, ty, tz) per facet, per midpoint."""
+    f = _u64(facets).reshape(-1, 3)
+    mids = traction_points_3d(m, f)
+    t = np.array([[traction_fn(tuple(p)) for p in fm] for fm in mids], dtype=np.float64)
+    out = np.empty(3 * m.nv)
+    lib().or_assemble_surface_traction_3d(m.nv, m.vertices, len(f), f, t.reshape(-1), out)
+    return out
+
+
+def assemble_robin_matrix_2d(m: Mesh, facets, gamma) -> Csc:
+    """boundary.rs:111-139."""
+    f = _u64(facets).reshape(-1, 2)
+    return Csc(lib().or_assemble_robin_matrix_2d(m.nv, m.vertices, len(f), f, float(gamma)))
+
+
+def p2_mass_products():
+    """mass.rs:190-250: [10, 10] reference products int phi_a phi_b."""
+    out = np.empty(100)
+    lib().or_p2_mass_products(out)
+    return out.reshape(10, 10)
+
+
+def assemble_elasticity_mass_p2_3d(nodes, cells, density, nthreads=0) -> Csc:
+    """mass.rs:136-180."""
+    v, c = _f64(nodes), _u64(cells)
+    return Csc(lib().or_assemble_elasticity_mass_p2_3d(len(v), v, len(c), c, float(density),
+                                                        nthreads))

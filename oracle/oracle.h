@@ -39,6 +39,10 @@ int or_boundary_facets_2d(uint64_t nc, const uint64_t* cells, uint64_t* facets_o
                           uint64_t* n_out);
 int or_boundary_facets_3d(uint64_t nc, const uint64_t* cells, uint64_t* facets_out,
                           uint64_t* n_out);
+/* with cell, local index and (3D) outward vertex order, same sort */
+int or_boundary_facets_full(int dim, const double* verts, uint64_t nc, const uint64_t* cells,
+                            uint64_t* verts_out, uint64_t* cell_out, uint64_t* local_out,
+                            uint64_t* n_out);
 int or_boundary_dofs_2d(uint64_t nc, const uint64_t* cells, uint64_t* dofs_out, uint64_t* n_out);
 int or_boundary_dofs_3d(uint64_t nc, const uint64_t* cells, uint64_t* dofs_out, uint64_t* n_out);
 double or_total_volume_2d(const double* verts, uint64_t nc, const uint64_t* cells);
@@ -146,6 +150,24 @@ int or_element_stresses(const double* verts, uint64_t nc, const uint64_t* cells,
                         double E, double nu, double* out);
 int or_vertex_von_mises(uint64_t nv, const double* verts, uint64_t nc, const uint64_t* cells,
                         const double* vm_cell, double* out);
+
+/* mass.rs:190-250 reference products int phi_a phi_b, out[a * 10 + b] */
+void or_p2_mass_products(double* out);
+/* mass.rs:136-180 on nodes [n][3], cells [nc][10] */
+or_csc* or_assemble_elasticity_mass_p2_3d(uint64_t n_nodes, const double* verts, uint64_t nc,
+                                          const uint64_t* cells, double density, int nthreads);
+
+/* ---- boundary (ds) integrals, boundary.rs ---- */
+/* Gauss points [nf][2][2] of each facet edge; the flux sampled there is g[nf][2] */
+int or_neumann_points_2d(const double* verts, uint64_t nf, const uint64_t* facets, double* xq);
+int or_assemble_neumann_2d(uint64_t nv, const double* verts, uint64_t nf, const uint64_t* facets,
+                           const double* g, double* rhs);
+/* edge midpoints [nf][3][3] of each facet triangle; the traction there is t[nf][3][3] */
+int or_traction_points_3d(const double* verts, uint64_t nf, const uint64_t* facets, double* mids);
+int or_assemble_surface_traction_3d(uint64_t nv, const double* verts, uint64_t nf,
+                                    const uint64_t* facets, const double* t, double* rhs);
+or_csc* or_assemble_robin_matrix_2d(uint64_t nv, const double* verts, uint64_t nf,
+                                    const uint64_t* facets, double gamma);
 
 #ifdef __cplusplus
 }

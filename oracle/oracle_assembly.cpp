@@ -272,6 +272,68 @@ or_csc* or_assemble_elasticity_mass_3d(uint64_t nv, const double* V, uint64_t nc
     return csc_from_trips(3 * nv, tr);
 }
 
+// mass.rs:190-250: exact reference-tetrahedron products of the P2 basis
+// (4 corners lambda_i (2 lambda_i - 1), then edges 01, 12, 02, 03, 13, 23 as
+// 4 lambda_i lambda_j), from prod m_i! / (sum m_i + 3)!, term by term.
+void or_p2_mass_products(double* out) {
+    static const double F[12] = {1.0,   1.0,    2.0,     6.0,      24.0,      120.0,
+                                 720.0, 5040.0, 40320.0, 362880.0, 3628800.0, 39916800.0};
+    struct Term {
+        std::array<int, 4> e;
+        double c;
+    };
+    std::vector<std::vector<Term>> T;
+    for (int i = 0; i < 4; ++i) {
+        std::array<int, 4> two = {0, 0, 0, 0}, one = {0, 0, 0, 0};
+        two[i] = 2;
+        one[i] = 1;
+        T.push_back({{two, 2.0}, {one, -1.0}});
+    }
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    for (auto& ed : E) {
+        std::array<int, 4> m = {0, 0, 0, 0};
+        m[ed[0]] = 1;
+        m[ed[1]] = 1;
+        T.push_back({{m, 4.0}});
+    }
+    for (int a = 0; a < 10; ++a)
+        for (int b = 0; b < 10; ++b) {
+            double acc = 0.0;
+            for (const Term& ta : T[a])
+                for (const Term& tb : T[b]) {
+                    std::array<int, 4> m;
+                    int deg = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        m[q] = ta.e[q] + tb.e[q];
+                        deg += m[q];
+                    }
+                    acc += ta.c * tb.c * F[m[0]] * F[m[1]] * F[m[2]] * F[m[3]] / F[deg + 3];
+                }
+            out[a * 10 + b] = acc;
+        }
+}
+
+// mass.rs:136-180: triplets per cell (cell order), density |det J| I_ab on
+// each displacement component.
+or_csc* or_assemble_elasticity_mass_p2_3d(uint64_t n_nodes, const double* V, uint64_t nc,
+                                          const uint64_t* C, double density, int nthreads) {
+    double I[100];
+    or_p2_mass_products(I);
+    auto tr = gen_trips(nc, nthreads, 300, [&](uint64_t c, std::vector<Trip>& out) {
+        const uint64_t* t = C + 10 * c;
+        Map3 m;
+        if (!affine3(V + 3 * t[0], V + 3 * t[1], V + 3 * t[2], V + 3 * t[3], m)) return;
+        const double det = std::fabs(m.det);
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) {
+                const double mv = density * det * I[i * 10 + j];
+                for (int a = 0; a < 3; ++a)
+                    out.push_back({(uint32_t)(3 * t[i] + a), (uint32_t)(3 * t[j] + a), mv});
+            }
+    });
+    return csc_from_trips(3 * n_nodes, tr);
+}
+
 // elasticity.rs:73-149.
 or_csc* or_assemble_elasticity_3d(uint64_t nv, const double* V, uint64_t nc, const uint64_t* C,
                                   double E, double nu, int nthreads) {
@@ -529,6 +591,99 @@ int or_vertex_von_mises(uint64_t nv, const double* V, uint64_t nc, const uint64_
     for (uint64_t i = 0; i < nv; ++i)
         if (vs[i] > 1e-15) out[i] /= vs[i];
     return 0;
+}
+
+// boundary.rs:8-46 (the 2-point Gauss rule of quadrature.rs:41-56 on [0,1]):
+// the flux is sampled by the caller at or_neumann_points_2d's points g[f][q].
+static double gauss2_point(int q) {
+    const double c = 1.0 / (2.0 * std::sqrt(3.0));
+    return q == 0 ? 0.5 - c : 0.5 + c;
+}
+int or_neumann_points_2d(const double* V, uint64_t nf, const uint64_t* facets, double* xq) {
+    for (uint64_t f = 0; f < nf; ++f) {
+        const double* v0 = V + 2 * facets[2 * f];
+        const double* v1 = V + 2 * facets[2 * f + 1];
+        for (int q = 0; q < 2; ++q) {
+            const double s = gauss2_point(q);
+            xq[4 * f + 2 * q] = (1.0 - s) * v0[0] + s * v1[0];
+            xq[4 * f + 2 * q + 1] = (1.0 - s) * v0[1] + s * v1[1];
+        }
+    }
+    return 0;
+}
+int or_assemble_neumann_2d(uint64_t nv, const double* V, uint64_t nf, const uint64_t* facets,
+                           const double* g, double* rhs) {
+    for (uint64_t i = 0; i < nv; ++i) rhs[i] = 0.0;
+    for (uint64_t f = 0; f < nf; ++f) {
+        const uint64_t a = facets[2 * f], b = facets[2 * f + 1];
+        const double* v0 = V + 2 * a;
+        const double* v1 = V + 2 * b;
+        const double dx = v1[0] - v0[0], dy = v1[1] - v0[1];
+        const double length = std::sqrt(dx * dx + dy * dy);
+        for (int q = 0; q < 2; ++q) {
+            const double s = gauss2_point(q);
+            const double w = 0.5 * length;
+            const double gv = g[2 * f + q];
+            rhs[a] += w * gv * (1.0 - s);
+            rhs[b] += w * gv * s;
+        }
+    }
+    return 0;
+}
+
+// boundary.rs:49-108: edge-midpoint rule on each facet triangle; the caller
+// samples the traction at or_traction_points_3d's midpoints t[f][m][3].
+int or_traction_points_3d(const double* V, uint64_t nf, const uint64_t* facets, double* mids) {
+    for (uint64_t f = 0; f < nf; ++f) {
+        const double* v[3] = {V + 3 * facets[3 * f], V + 3 * facets[3 * f + 1],
+                              V + 3 * facets[3 * f + 2]};
+        for (int m = 0; m < 3; ++m)
+            for (int a = 0; a < 3; ++a)
+                mids[9 * f + 3 * m + a] = 0.5 * (v[m][a] + v[(m + 1) % 3][a]);
+    }
+    return 0;
+}
+int or_assemble_surface_traction_3d(uint64_t nv, const double* V, uint64_t nf,
+                                    const uint64_t* facets, const double* t, double* rhs) {
+    for (uint64_t i = 0; i < 3 * nv; ++i) rhs[i] = 0.0;
+    for (uint64_t f = 0; f < nf; ++f) {
+        const uint64_t* fv = facets + 3 * f;
+        const double* v0 = V + 3 * fv[0];
+        const double* v1 = V + 3 * fv[1];
+        const double* v2 = V + 3 * fv[2];
+        const double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        const double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        const double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                             e1[0] * e2[1] - e1[1] * e2[0]};
+        const double area = 0.5 * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        const double w = area / 3.0;
+        for (int m = 0; m < 3; ++m) {
+            const double* tm = t + 9 * f + 3 * m;
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) rhs[3 * fv[k] + a] += (w / 3.0) * tm[a];
+        }
+    }
+    return 0;
+}
+
+// boundary.rs:111-139: gamma times the 1D P1 mass [L/3 L/6; L/6 L/3] of each
+// facet edge, as triplets summed per position.
+or_csc* or_assemble_robin_matrix_2d(uint64_t nv, const double* V, uint64_t nf,
+                                    const uint64_t* facets, double gamma) {
+    std::vector<Trip> tr;
+    tr.reserve(4 * nf);
+    for (uint64_t f = 0; f < nf; ++f) {
+        const uint64_t n0 = facets[2 * f], n1 = facets[2 * f + 1];
+        const double* v0 = V + 2 * n0;
+        const double* v1 = V + 2 * n1;
+        const double dx = v1[0] - v0[0], dy = v1[1] - v0[1];
+        const double length = std::sqrt(dx * dx + dy * dy);
+        tr.push_back({(uint32_t)n0, (uint32_t)n0, gamma * length / 3.0});
+        tr.push_back({(uint32_t)n0, (uint32_t)n1, gamma * length / 6.0});
+        tr.push_back({(uint32_t)n1, (uint32_t)n0, gamma * length / 6.0});
+        tr.push_back({(uint32_t)n1, (uint32_t)n1, gamma * length / 3.0});
+    }
+    return csc_from_trips(nv, tr);
 }
 
 }  // extern "C"

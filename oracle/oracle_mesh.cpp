@@ -322,6 +322,83 @@ int or_boundary_facets_3d(uint64_t nc, const uint64_t* cells, uint64_t* facets_o
     return 0;
 }
 
+// mesh.rs:79-113 / 145-210 with each facet's cell, local index and vertex
+// order: 2D edges as the sorted pair (the HashMap key), 3D faces in the
+// cell's local order, swapped to [f0, f2, f1] when the normal
+// (f1 - f0) x (f2 - f0) points towards the opposite vertex.  Sorted by the
+// sorted vertex tuple (the reference's HashMap order is random).
+int or_boundary_facets_full(int dim, const double* V, uint64_t nc, const uint64_t* cells,
+                            uint64_t* verts_out, uint64_t* cell_out, uint64_t* local_out,
+                            uint64_t* n_out) {
+    struct F {
+        std::array<uint64_t, 3> key;
+        uint64_t cell, local;
+    };
+    const int K = dim + 1;
+    std::vector<F> all;
+    all.reserve(K * nc);
+    for (uint64_t c = 0; c < nc; ++c) {
+        const uint64_t* t = cells + K * c;
+        for (int a = 0; a < K; ++a) {
+            std::array<uint64_t, 3> k = {0, 0, 0};
+            int m = 0;
+            if (dim == 2) {  // local facet a: (1,2), (2,0), (0,1)
+                k[0] = t[(a + 1) % 3];
+                k[1] = t[(a + 2) % 3];
+                m = 2;
+            } else {
+                for (int b = 0; b < 4; ++b)
+                    if (b != a) k[m++] = t[b];
+            }
+            std::sort(k.begin(), k.begin() + m);
+            all.push_back({k, c, (uint64_t)a});
+        }
+    }
+    std::sort(all.begin(), all.end(), [](const F& x, const F& y) {
+        return x.key != y.key ? x.key < y.key : x.cell < y.cell;
+    });
+    uint64_t n = 0;
+    for (size_t i = 0; i < all.size();) {
+        size_t j = i;
+        while (j < all.size() && all[j].key == all[i].key) ++j;
+        if (j - i == 1) {
+            const F& f = all[i];
+            if (verts_out) {
+                if (dim == 2) {
+                    verts_out[2 * n] = f.key[0];
+                    verts_out[2 * n + 1] = f.key[1];
+                } else {
+                    const uint64_t* t = cells + 4 * f.cell;
+                    uint64_t fv[3];
+                    int m = 0;
+                    for (int b = 0; b < 4; ++b)
+                        if (b != (int)f.local) fv[m++] = t[b];
+                    const double* va = V + 3 * fv[0];
+                    const double* vb = V + 3 * fv[1];
+                    const double* vc = V + 3 * fv[2];
+                    const double* vo = V + 3 * t[f.local];
+                    const double ab[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+                    const double ac[3] = {vc[0] - va[0], vc[1] - va[1], vc[2] - va[2]};
+                    const double nrm[3] = {ab[1] * ac[2] - ab[2] * ac[1],
+                                           ab[2] * ac[0] - ab[0] * ac[2],
+                                           ab[0] * ac[1] - ab[1] * ac[0]};
+                    const double tf[3] = {va[0] - vo[0], va[1] - vo[1], va[2] - vo[2]};
+                    const double dot = nrm[0] * tf[0] + nrm[1] * tf[1] + nrm[2] * tf[2];
+                    verts_out[3 * n] = fv[0];
+                    verts_out[3 * n + 1] = dot >= 0.0 ? fv[1] : fv[2];
+                    verts_out[3 * n + 2] = dot >= 0.0 ? fv[2] : fv[1];
+                }
+            }
+            if (cell_out) cell_out[n] = f.cell;
+            if (local_out) local_out[n] = f.local;
+            ++n;
+        }
+        i = j;
+    }
+    *n_out = n;
+    return 0;
+}
+
 // poisson.rs:266-287: facet vertices, sorted and deduplicated.
 static int boundary_dofs(int dim, uint64_t nc, const uint64_t* cells, uint64_t* out,
                          uint64_t* n_out) {

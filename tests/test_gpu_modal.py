@@ -49,6 +49,20 @@ def test_elasticity_mass_and_folds_bitwise():
     assert same_csc(fh.fold_dirichlet(K, dup, True), O.fold_dirichlet(Ko, fixed, True))
 
 
+def test_p2_vector_mass_bitwise():
+    om = O.box_beam(2.0, 1.0, 0.5, 5, 3, 2)
+    p2 = O.P2(om)
+    X, cd = p2.dof_coords(), p2.cell_dofs()
+    M, Mo = fh.assemble_elasticity_mass_p2_3d(X, cd, 7.5), O.assemble_elasticity_mass_p2_3d(X, cd, 7.5)
+    assert same_csc(M, Mo)
+    assert M.nnz() == len(Mo.export()[2])
+    nodes = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0, 0], [0.5, 0.5, 0], [0, 0.5, 0],
+             [0, 0, 0.5], [0.5, 0, 0.5], [0, 0.5, 0.5]]  # mass.rs:296-330
+    M1 = fh.assemble_elasticity_mass_p2_3d(nodes, [list(range(10))], 1.0)
+    one = np.ones(30)
+    assert abs(one @ M1.spmv(one) - 0.5) < 1e-13
+
+
 def test_cantilever_natural_frequencies():  # modal_tests.rs:9-86
     gm, om, K, M, Ko, Mo, fixed = beam(16, 2, 2)
     modes = fh.solve_vibration_modes(K, M, fixed, 3, 30, 1e-5)

@@ -539,3 +539,41 @@ def test_vibration_modes_vs_shift_invert_lanczos():
     Ms = M.to_scipy()
     G = sh @ (Ms @ sh.T)
     assert np.allclose(G / G[0, 0], np.eye(3), atol=1e-6) and not sh[:, fx].any()
+
+
+def test_p2_mass_products_closed_forms():  # mass.rs:270-289
+    P = O.p2_mass_products()
+    for (a, b), v in {(0, 0): 1 / 420, (4, 4): 4 / 315, (0, 1): 1 / 2520, (0, 4): -1 / 630,
+                      (4, 6): 2 / 315, (4, 9): 1 / 315}.items():
+        assert abs(P[a, b] - v) < 1e-14
+    rows = P.sum(axis=1)
+    assert np.allclose(rows[:4], -1 / 120, rtol=0, atol=1e-14)
+    assert np.allclose(rows[4:], 1 / 30, rtol=0, atol=1e-14)
+    assert abs(P.sum() - 1 / 6) < 1e-13 and np.array_equal(P, P.T)
+
+
+def test_p2_vector_mass_one_tet_kat():  # mass.rs:296-330
+    nodes = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0, 0], [0.5, 0.5, 0], [0, 0.5, 0],
+             [0, 0, 0.5], [0.5, 0, 0.5], [0, 0.5, 0.5]]
+    M = O.assemble_elasticity_mass_p2_3d(nodes, [list(range(10))], 1.0).to_scipy()
+    one = np.ones(30)
+    assert abs(one @ (M @ one) - 0.5) < 1e-13
+    assert abs(M[0, 0] - 1 / 420) < 1e-13
+    assert M.nnz == 10 * 10 * 3  # equal components only
+
+
+def test_p2_vector_mass_integrates_quadratics():
+    """sum_i (M u)_i = rho int u for the P2 interpolant of a quadratic field
+    (sum_i phi_i = 1), component by component, on a box of 2 x 1 x 0.5."""
+    m = O.box_beam(2.0, 1.0, 0.5, 3, 2, 2)
+    p2 = O.P2(m)
+    X = p2.dof_coords()
+    M = O.assemble_elasticity_mass_p2_3d(X, p2.cell_dofs(), 7.0).to_scipy()
+    u = np.zeros((len(X), 3))
+    u[:, 0] = X[:, 0] ** 2
+    u[:, 1] = X[:, 1] * X[:, 2]
+    u[:, 2] = 1.0 + X[:, 2] ** 2
+    tot = (M @ u.reshape(-1)).reshape(-1, 3).sum(axis=0)
+    vol = 2.0 * 1.0 * 0.5
+    exact = 7.0 * np.array([vol * 4 / 3, vol * 0.5 * 0.25, vol * (1 + 0.25 / 3)])
+    assert np.allclose(tot, exact, rtol=1e-13, atol=0)
