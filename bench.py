@@ -174,11 +174,12 @@ def main():
     t_hi = sm.spmv_avg_ms * sm.spmv_timed
     t_lo = sm.spmv32_avg_ms * sm.spmv32_timed
     if t_hi >= t_lo:
-        kname, kbytes, kms = ("k_cg_spmv<3,false,false,false> (FP64 values, "
-                              "fused p-update and p.Ap)"), sm.spmv_bytes, sm.spmv_avg_ms
+        kname, kbytes, kms = ("k_cg_spmv<3,false,false,false,false> (FP64 values, p = z + beta p "
+                              "formed per gathered column, p.Ap)"), sm.spmv_bytes, sm.spmv_avg_ms
     else:
-        kname, kbytes, kms = ("k_cg_spmv<3,false,true,true> (FP32 values, difference form, "
-                              "fused p-update and p.Ap)"), sm.spmv32_bytes, sm.spmv32_avg_ms
+        kname, kbytes, kms = ("k_cg_spmv<3,false,true,true,false> (FP32 values, difference form, "
+                              "p = z + beta p per gathered column, p.Ap)"), sm.spmv32_bytes, \
+            sm.spmv32_avg_ms
     achieved = gbps(kbytes, kms)
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -127,6 +127,7 @@ struct CgMat {
     const uint32_t* cols;
     const double* v64;
     const float* v32;
+    const uint32_t* t64;  // (last value, column) records, see fh_matrix::tail64
     const double* S64;  // block row sums, lane-interleaved [slice][bs*bs][64]
     const float* S32;
     uint32_t nrows;
@@ -398,35 +399,36 @@ __device__ __forceinline__ void ld_pair(const T* p, T& a, T& b) {
     b = v.y;
 }
 
+// one (last value, column) tail record: 12 bytes
+template <bool NT>
+__device__ __forceinline__ void ld_tail(const uint32_t* p, double& v, uint32_t& c) {
+    typedef uint32_t w3 __attribute__((ext_vector_type(3)));
+    const w3 t = ld_s<NT>(reinterpret_cast<const w3*>(p));
+    v = __hiloint2double((int)t.y, (int)t.x);
+    c = t.z;
+}
+
 // Software-pipelined form of row_product for the CG loop (fused-multiply-add
 // order): U stored blocks per step, the column indices of the next step
 // loaded before this step's gathers.  No per-lane length test: padded SELL
 // entries hold zero values and the row's own column (or column 0 past the
 // last row), so they add exact zeros.
-// With NP = 2 the slice's steps are split in two halves (part 0: the first,
-// rounded up to whole steps of U) for two waves to run side by side.
-template <int BS, int U, class TM, bool DIFF, bool NT, int NP = 1, class G>
+template <int BS, int U, class TM, bool DIFF, bool NT, class G>
 __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __restrict__ vals,
                                                  uint32_t s, uint32_t lane, const double* xr,
-                                                 G&& gather, double* acc, uint32_t part = 0) {
+                                                 G&& gather, double* acc) {
     constexpr int B2 = BS * BS;
     const uint32_t base = A.soff[s], w = (A.soff[s + 1] - base) / SLICE;
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    uint32_t k0 = 0, k1 = w;
-    if constexpr (NP == 2) {
-        const uint32_t mid = min(w, ((w + 1) / 2 + U - 1) / U * U);
-        k0 = part ? mid : 0;
-        k1 = part ? w : mid;
-    }
-    if (k1 <= k0) return;
+    if (w == 0) return;
     const uint32_t* cp = A.cols + (uint64_t)base + lane;
     const TM* vb = vals + (uint64_t)base * B2;  // step k: vb + k * SLICE * B2, see vpos
-    const uint32_t last = k1 - 1;
+    const uint32_t last = w - 1;
     uint32_t c[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min(k0 + u, last) * SLICE);
-    for (uint32_t k = k0; k < k1; k += U) {
+    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min((uint32_t)u, last) * SLICE);
+    for (uint32_t k = 0; k < w; k += U) {
         uint32_t cn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -449,8 +451,8 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             gather(c[u], xg[u]);
-            // steps past the range re-read its last entry: add zero
-            const bool ok = u == 0 || k + u < k1;
+            // steps past the slice width re-read its last entry: add zero
+            const bool ok = u == 0 || k + u < w;
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
                 const double d = DIFF ? xg[u][q] - xr[q] : xg[u][q];
@@ -469,6 +471,46 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
     }
 }
 
+// The same for FP64 3x3 blocks, one block per step, with the block's last
+// value and column index in one 12-byte tail record loaded a step ahead: one
+// memory instruction fewer per block (measured -1.5% at C1; the FP32 and
+// scalar products measured 7-8% slower with tails, so they keep the form above).
+template <bool DIFF, class G>
+__device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, uint32_t lane,
+                                                  const double* xr, G&& gather, double* acc) {
+    const uint32_t base = A.soff[s], w = (A.soff[s + 1] - base) / SLICE;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = 0.0;
+    if (w == 0) return;
+    const uint32_t* tp = A.t64 + ((uint64_t)base + lane) * 3;  // step k: tp + k * SLICE * 3
+    const double* vb = A.v64 + (uint64_t)base * 9;              // step k: vb + k * SLICE * 9
+    const uint32_t last = w - 1;
+    double vl;
+    uint32_t c;
+    ld_tail<true>(tp, vl, c);
+    for (uint32_t k = 0; k < w; ++k) {
+        double vn;
+        uint32_t cn;
+        ld_tail<true>(tp + (uint64_t)min(k + 1, last) * SLICE * 3, vn, cn);
+        double v[9];
+        const double* vk = vb + (uint64_t)k * SLICE * 9;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) ld_pair<true>(vk + 128 * pp + 2 * lane, v[2 * pp], v[2 * pp + 1]);
+        v[8] = vl;
+        double xg[3];
+        gather(c, xg);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (DIFF) xg[q] -= xr[q];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[a] = fma(v[a * 3 + q], xg[q], acc[a]);
+        c = cn;
+        vl = vn;
+    }
+}
+
 // LOW: the product reads the FP32 copy of the matrix (difference form).  The
 // host picks the instantiation per chunk of iterations; the FP64 one raises
 // st->want_low when the relaxation threshold is crossed.
@@ -481,10 +523,7 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 // component).  The kernel forms p_new = z + beta p_old on the fly for every
 // gathered column and for its own rows; it writes only Ap: the update kernel
 // rebuilds the same p_new (same beta, same arithmetic) and stores the pairs.
-// SP = 2: two waves per slice, each over half of its stored blocks, combined
-// through LDS (a block holds two slices): twice the waves in flight for the
-// same rows.
-template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false, int SP = 1>
+template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false>
 __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ zp,
                                                  double* __restrict__ Ap, CgState* st,
                                                  const double2* __restrict__ red_in, int n_in,
@@ -492,9 +531,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
                                                  const double* __restrict__ fl_in,
                                                  double* __restrict__ red_out) {
     static_assert(DIFF || !LOW, "the FP32 operator is only used in difference form");
-    static_assert(SP == 1 || !ORD, "the ordered product runs one wave per slice");
     __shared__ double sh[4];
-    __shared__ double half_acc[SP == 2 ? 2 * BS * SLICE : 1];
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
@@ -632,16 +669,13 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
 
     const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
     // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
-    const unsigned wv = threadIdx.x >> 6;
-    const unsigned s = __builtin_amdgcn_readfirstlane(SP == 2 ? blk * 2 + (wv >> 1) : blk * 4 + wv);
-    const unsigned part = SP == 2 ? (wv & 1) : 0;
+    const unsigned s = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
     const unsigned lane = threadIdx.x & 63;
     double dot = 0.0;
-    double acc[BS], xr[BS];
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        double xz[BS];
+        double xr[BS], xz[BS], acc[BS];
 #pragma unroll
         for (int q = 0; q < BS; ++q) {
             const uint64_t i = ((uint64_t)s * BS + q) * SLICE + lane;
@@ -663,29 +697,16 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
         // measured 2-4% faster than two), two for FP32 ones, four scalars
         constexpr int U = BS == 1 ? 4 : (LOW ? 2 : 1);
         if constexpr (LOW) {
-            row_product_pipe<BS, U, float, true, false, SP>(A, A.v32, s, lane, xo, gather, acc,
-                                                            part);
-            if (part == 0) add_rowsum<BS>(A.S32, s, lane, xo, acc);
+            row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
+            add_rowsum<BS>(A.S32, s, lane, xo, acc);
         } else if constexpr (ORD) {
             row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
+        } else if constexpr (BS == 3) {
+            row_product_tail3<DIFF>(A, s, lane, xo, gather, acc);
         } else {
-            row_product_pipe<BS, U, double, DIFF, true, SP>(A, A.v64, s, lane, xo, gather, acc,
-                                                            part);
-            if (DIFF && part == 0) add_rowsum<BS>(A.S64, s, lane, xo, acc);
+            row_product_pipe<BS, U, double, DIFF, true>(A, A.v64, s, lane, xo, gather, acc);
+            if (DIFF) add_rowsum<BS>(A.S64, s, lane, xo, acc);
         }
-    }
-    if constexpr (SP == 2) {
-        // the second half hands its sums to the first
-        double* h = half_acc + (wv >> 1) * BS * SLICE + lane;
-        if (part == 1 && s < A.nslices)
-#pragma unroll
-            for (int a = 0; a < BS; ++a) h[a * SLICE] = acc[a];
-        __syncthreads();
-        if (part == 0 && s < A.nslices)
-#pragma unroll
-            for (int a = 0; a < BS; ++a) acc[a] += h[a * SLICE];
-    }
-    if (part == 0 && s < A.nslices) {
         // every lane stores (rows past the last hold zeros): full 512-byte stores
 #pragma unroll
         for (int a = 0; a < BS; ++a) {
@@ -889,8 +910,8 @@ __global__ __launch_bounds__(256) void k_true_res(CgMat A, const double* __restr
 
 static CgMat cgmat(const fh_matrix* a) {
     const Pattern& p = *a->pat;
-    return {p.slice_off.p, p.row_len.p, p.sell_cols.p, a->vals.p, a->vals32.p,
-            a->rowsum64.p, a->rowsum32.p, p.nrows, p.nslices};
+    return {p.slice_off.p, p.row_len.p, p.sell_cols.p, a->vals.p,     a->vals32.p,
+            a->tail64.p,     a->rowsum64.p,  a->rowsum32.p,   p.nrows, p.nslices};
 }
 
 static unsigned spmv_grid(const fh_matrix* a) { return std::max(1u, cdiv(a->pat->nslices, 4)); }
@@ -950,6 +971,32 @@ void ensure_rowsum(fh_matrix* a, bool f32) {
     }
 }
 
+// (last value, column) record of every SELL entry e of a 3x3-block matrix
+// (slices start at multiples of 64, so e's step base is e rounded down to 64
+// and its lane e mod 64)
+__global__ void k_make_tail(const double* __restrict__ vals, const uint32_t* __restrict__ cols,
+                            uint64_t n, uint32_t* __restrict__ tail) {
+    const uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t b =
+        (uint64_t)__double_as_longlong(vals[vpos(e & ~63ull, 8, (uint32_t)(e & 63), 9)]);
+    tail[3 * e] = (uint32_t)b;
+    tail[3 * e + 1] = (uint32_t)(b >> 32);
+    tail[3 * e + 2] = cols[e];
+}
+
+void ensure_tail(fh_matrix* a) {
+    if (a->tail64.p || a->bs != 3) return;
+    DeviceCtx& d = device(a->dev);
+    const uint64_t n = a->pat->sell_entries;
+    a->tail64.alloc(a->dev, std::max<uint64_t>(3 * n, 1));
+    if (n) {
+        hipLaunchKernelGGL(k_make_tail, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0, d.stream,
+                           a->vals.p, a->pat->sell_cols.p, n, a->tail64.p);
+        FH_LAUNCH_CHECK("make_tail");
+    }
+}
+
 // pinned host mirror of the state, one per device
 static CgState* host_state(int dev) {
     static std::mutex mu;
@@ -1005,6 +1052,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     fh_matrix* am = const_cast<fh_matrix*>(a);
     if (uses32) ensure_vals32(am);
     if (uses32 || diff64) ensure_rowsum(am, uses32);
+    if (!ORD && BS == 3) ensure_tail(am);
     const CgMat A = cgmat(a);
 
     // zp: (z, p) pairs at each lane-interleaved index
@@ -1029,7 +1077,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     else fill_f64(xl.p, npad, 0.0, s);
     fill_f64(zp.p, 2 * (size_t)npad, 0.0, s);
     double* x = xl.p;
-    DBuf<double> redA(dev, std::max(gs, (unsigned)cdiv(nsl, 2))), bb(dev, gs), fl(dev, gs);
+    DBuf<double> redA(dev, gs), bb(dev, gs), fl(dev, gs);
     DBuf<double2> redB(dev, std::max(gs, gu));
 
     CgState* hs = host_state(dev);
@@ -1068,23 +1116,14 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
     // the initial residual or a replacement, where the exact product restarts it
     const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
-    // FH_CG_SPLIT=2: two waves per slice in the CG product (not in ordered mode)
-    const bool split = !ORD && env_int("FH_CG_SPLIT", 1) == 2;
-    const unsigned gsp = split ? std::max(1u, cdiv(nsl, 2)) : gs;  // CG spmv grid
     auto launch_chunk = [&](int nin_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m], s));
             const int nin = m == 0 ? nin_first : (int)gu;
             const bool rec = aprec && !(m == 0 && nin_first == (int)gs);
-#define FH_SPMV_SP(D, L, R, SPV)                                                                 \
-    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R, SPV>), dim3(gsp), dim3(256), 0, s, A, zp.p,  \
-                       Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p)
-#define FH_SPMV(D, L, R)                                      \
-    do {                                                      \
-        if constexpr (ORD) FH_SPMV_SP(D, L, R, 1);            \
-        else if (split) FH_SPMV_SP(D, L, R, 2);               \
-        else FH_SPMV_SP(D, L, R, 1);                          \
-    } while (0)
+#define FH_SPMV(D, L, R)                                                                    \
+    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, A, zp.p, Ap.p, \
+                       st.p, redB.p, nin, bb.p, fl.p, redA.p)
             if constexpr (DIFF) {
                 if (rec) {
                     if (lowk) FH_SPMV(true, true, true);
@@ -1099,10 +1138,9 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                 FH_SPMV(false, false, false);
             }
 #undef FH_SPMV
-#undef FH_SPMV_SP
             if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m + 1], s));
             hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, npad, x, r.p,
-                               Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gsp, redB.p);
+                               Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gs, redB.p);
         }
     };
     uint64_t launched = 0;

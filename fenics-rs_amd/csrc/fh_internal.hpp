@@ -78,6 +78,9 @@ struct fh_matrix {
     std::shared_ptr<fh::Pattern> pat;
     fh::DBuf<double> vals;   // SELL layout, [entries][bs*bs] interleaved by lane
     fh::DBuf<float> vals32;  // FP32 copy (lazily made)
+    // 3x3 blocks: per SELL entry, the block's last value and its column in one
+    // record (the FP64 CG product loads both with one instruction) [entries][3 words]
+    fh::DBuf<uint32_t> tail64;
     fh::DBuf<double> rowsum64;  // block row sums, lane-interleaved [slice][bs*bs][64] (difference form)
     fh::DBuf<float> rowsum32;   // FP32 copy of rowsum64
     fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
@@ -146,6 +149,7 @@ uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
 void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStream_t s);
 void inv_diag_f64(const fh_matrix* a, double* out, hipStream_t s);
 void ensure_rowsum(fh_matrix* a, bool f32);
+void ensure_tail(fh_matrix* a);
 
 // Solve (fh_cg.hip)
 struct SolveArgs {
