@@ -156,8 +156,8 @@ def main():
     log(f"fp64 reference semantics (recursive stop): {sref.iterations} it, {msref:.1f} ms/step, "
         f"recursive rel res {sref.rel_residual:.2e}, true rel res {sref.true_rel_residual:.2e}")
 
-    # ---------------- SpMV kernel durations: one more solve of each kind with a HIP
-    # event pair around every SpMV launch on the library's stream
+    # ---------------- SpMV kernel durations: one more solve of each kind, timed by
+    # start/stop events carried by every SpMV dispatch on the library's stream
     os.environ["FH_TIME_SPMV"] = "1"
     sm = step(fh.MIXED, u_mix)
     st64 = step(fh.FP64_TRUE, u_64)

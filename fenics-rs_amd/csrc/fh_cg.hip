@@ -72,6 +72,8 @@
 #include <mutex>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "fh_internal.hpp"
 
 namespace fh {
@@ -1103,7 +1105,9 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
 
     const int M = std::max(1, env_int("FH_CG_CHUNK", 16));
     const bool use_graph = env_int("FH_CG_GRAPH", 1) != 0;
-    // optional per-launch timing of the SpMV kernel with HIP events on this stream
+    // optional per-launch timing of the SpMV kernel: a start and a stop event
+    // carried by the kernel's own dispatch (hipExtLaunchKernelGGL), so the pair
+    // brackets its execution on this stream and nothing else
     const bool time_spmv = env_int("FH_TIME_SPMV", 0) != 0;
     const bool trace = env_int("FH_CG_TRACE", 0) != 0;  // per-chunk state on stderr
     std::vector<hipEvent_t> ev;
@@ -1118,12 +1122,20 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
     auto launch_chunk = [&](int nin_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
-            if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m], s));
             const int nin = m == 0 ? nin_first : (int)gu;
             const bool rec = aprec && !(m == 0 && nin_first == (int)gs);
-#define FH_SPMV(D, L, R)                                                                    \
-    hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, A, zp.p, Ap.p, \
-                       st.p, redB.p, nin, bb.p, fl.p, redA.p)
+            hipEvent_t e0 = time_spmv ? ev[2 * m] : nullptr, e1 = time_spmv ? ev[2 * m + 1] : nullptr;
+#define FH_SPMV(D, L, R)                                                                         \
+    do {                                                                                         \
+        if (time_spmv)                                                                           \
+            hipExtLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, e0,  \
+                                  e1, 0, A, (const double*)zp.p, Ap.p, st.p,                     \
+                                  (const double2*)redB.p, nin, (const double*)bb.p,              \
+                                  (const double*)fl.p, redA.p);                                  \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, A,      \
+                               zp.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);               \
+    } while (0)
             if constexpr (DIFF) {
                 if (rec) {
                     if (lowk) FH_SPMV(true, true, true);
@@ -1138,7 +1150,6 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                 FH_SPMV(false, false, false);
             }
 #undef FH_SPMV
-            if (time_spmv) FH_HIP(hipEventRecord(ev[2 * m + 1], s));
             hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, npad, x, r.p,
                                Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gs, redB.p);
         }

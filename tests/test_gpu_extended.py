@@ -165,6 +165,8 @@ def test_c4_full_size_batched_fp64_and_mixed():
     for prec in (fh.FP64, fh.MIXED):
         us, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, max_iter=40000,
                                                   precision=prec)
+        print(f"C4 precision {prec}: {st.wall_ms:.1f} ms, column iterations {its.min()}-{its.max()}",
+              flush=True)
         assert its.min() > 0 and st.status == fh.FH_OK
         sols[prec] = us
         for i in picks:
@@ -196,7 +198,9 @@ def test_c2_full_size_p2_solve_fp64_and_mixed():
     Kb, bb = fh.apply_dirichlet_bc(K, b, fh.DirichletBC(bd, np.zeros(len(bd))))
     x64, s64 = Kb.solve(bb, tol=1e-10, max_iter=20000, precision=fh.FP64_TRUE)
     xmx, smx = Kb.solve(bb, tol=1e-10, max_iter=20000, precision=fh.MIXED)
-    for st in (s64, smx):
+    for name, st in (("fp64", s64), ("mixed", smx)):
+        print(f"C2 {name}: {st.iterations} iterations ({st.iterations_low} on FP32 values), "
+              f"{st.wall_ms:.1f} ms, true rel. residual {st.true_rel_residual:.2e}", flush=True)
         assert st.true_rel_residual < max(1e-10, 2.0 * st.residual_floor)
     assert smx.iterations_low > 0
     assert np.linalg.norm(xmx - x64) / np.linalg.norm(x64) < 1e-8
