@@ -401,11 +401,13 @@ __device__ __forceinline__ void ld_pair(const T* p, T& a, T& b) {
     b = v.y;
 }
 
-// one (last value, column) tail record: 12 bytes
+// words per (last value, column) tail record: 3 (12 bytes, one dwordx3 load)
+constexpr int TAILW = 3;
+
 template <bool NT>
 __device__ __forceinline__ void ld_tail(const uint32_t* p, double& v, uint32_t& c) {
-    typedef uint32_t w3 __attribute__((ext_vector_type(3)));
-    const w3 t = ld_s<NT>(reinterpret_cast<const w3*>(p));
+    typedef uint32_t wv __attribute__((ext_vector_type(TAILW)));
+    const wv t = ld_s<NT>(reinterpret_cast<const wv*>(p));
     v = __hiloint2double((int)t.y, (int)t.x);
     c = t.z;
 }
@@ -484,7 +486,7 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, ui
 #pragma unroll
     for (int a = 0; a < 3; ++a) acc[a] = 0.0;
     if (w == 0) return;
-    const uint32_t* tp = A.t64 + ((uint64_t)base + lane) * 3;  // step k: tp + k * SLICE * 3
+    const uint32_t* tp = A.t64 + ((uint64_t)base + lane) * TAILW;  // step k: + k * SLICE * TAILW
     const double* vb = A.v64 + (uint64_t)base * 9;              // step k: vb + k * SLICE * 9
     const uint32_t last = w - 1;
     double vl;
@@ -493,7 +495,7 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, ui
     for (uint32_t k = 0; k < w; ++k) {
         double vn;
         uint32_t cn;
-        ld_tail<true>(tp + (uint64_t)min(k + 1, last) * SLICE * 3, vn, cn);
+        ld_tail<true>(tp + (uint64_t)min(k + 1, last) * SLICE * TAILW, vn, cn);
         double v[9];
         const double* vk = vb + (uint64_t)k * SLICE * 9;
 #pragma unroll
@@ -982,16 +984,17 @@ __global__ void k_make_tail(const double* __restrict__ vals, const uint32_t* __r
     if (e >= n) return;
     const uint64_t b =
         (uint64_t)__double_as_longlong(vals[vpos(e & ~63ull, 8, (uint32_t)(e & 63), 9)]);
-    tail[3 * e] = (uint32_t)b;
-    tail[3 * e + 1] = (uint32_t)(b >> 32);
-    tail[3 * e + 2] = cols[e];
+    tail[TAILW * e] = (uint32_t)b;
+    tail[TAILW * e + 1] = (uint32_t)(b >> 32);
+    tail[TAILW * e + 2] = cols[e];
+    if constexpr (TAILW == 4) tail[TAILW * e + 3] = 0u;
 }
 
 void ensure_tail(fh_matrix* a) {
     if (a->tail64.p || a->bs != 3) return;
     DeviceCtx& d = device(a->dev);
     const uint64_t n = a->pat->sell_entries;
-    a->tail64.alloc(a->dev, std::max<uint64_t>(3 * n, 1));
+    a->tail64.alloc(a->dev, std::max<uint64_t>(TAILW * n, 1));
     if (n) {
         hipLaunchKernelGGL(k_make_tail, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0, d.stream,
                            a->vals.p, a->pat->sell_cols.p, n, a->tail64.p);
