@@ -366,21 +366,22 @@ __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restr
 }
 
 // ---------------------------------------------------------------- CG kernels
+template <int NT = 256>
 __device__ __forceinline__ double2 reduce_pairs(const double2* __restrict__ p, int m, double* sh) {
     double a = 0.0, b = 0.0;
     int i = threadIdx.x;
-    for (; i + 768 < m; i += 1024) {  // four independent loads in flight
-        const double2 u = p[i], v = p[i + 256], w = p[i + 512], t = p[i + 768];
+    for (; i + 3 * NT < m; i += 4 * NT) {  // four independent loads in flight
+        const double2 u = p[i], v = p[i + NT], w = p[i + 2 * NT], t = p[i + 3 * NT];
         a += (u.x + v.x) + (w.x + t.x);
         b += (u.y + v.y) + (w.y + t.y);
     }
-    for (; i < m; i += 256) {
+    for (; i < m; i += NT) {
         const double2 v = p[i];
         a += v.x;
         b += v.y;
     }
-    a = fhd::block_sum256(a, sh);
-    b = fhd::block_sum256(b, sh);
+    a = fhd::block_sum<NT>(a, sh);
+    b = fhd::block_sum<NT>(b, sh);
     return make_double2(a, b);
 }
 
@@ -515,6 +516,18 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, ui
     }
 }
 
+// Block shapes of the two CG kernels.  Every block of each kernel first
+// reduces all block partials of the other one, so the partial counts are kept
+// small: CG_SPB slices (waves) per k_cg_spmv block, UPD_T threads per
+// k_cg_update block and at most UPD_MAX update blocks (measured against 4 /
+// 256 / 1024: -1% per iteration at the 1M-dof cantilever, -6.5% on a 4.2M-row
+// scalar Poisson matrix, where the partial reads had grown to ~400 MB per
+// iteration).
+constexpr int CG_SPB = 8;
+constexpr int CG_T = 64 * CG_SPB;
+constexpr int UPD_T = 1024;
+constexpr int UPD_MAX = 256;
+
 // LOW: the product reads the FP32 copy of the matrix (difference form).  The
 // host picks the instantiation per chunk of iterations; the FP64 one raises
 // st->want_low when the relaxation threshold is crossed.
@@ -528,19 +541,19 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, ui
 // gathered column and for its own rows; it writes only Ap: the update kernel
 // rebuilds the same p_new (same beta, same arithmetic) and stores the pairs.
 template <int BS, bool ORD, bool DIFF, bool LOW, bool REC = false>
-__global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restrict__ zp,
+__global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, const double* __restrict__ zp,
                                                  double* __restrict__ Ap, CgState* st,
                                                  const double2* __restrict__ red_in, int n_in,
                                                  const double* __restrict__ bb_in,
                                                  const double* __restrict__ fl_in,
                                                  double* __restrict__ red_out) {
     static_assert(DIFF || !LOW, "the FP32 operator is only used in difference form");
-    __shared__ double sh[4];
+    __shared__ double sh[CG_SPB];
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
     const double tol = st->tol;
-    const double2 rr_rz = reduce_pairs(red_in, n_in, sh);
+    const double2 rr_rz = reduce_pairs<CG_T>(red_in, n_in, sh);
     const double rr = rr_rz.x, rz = rr_rz.y;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const bool low_prev = j > 1 ? st->low[(j - 1) & 1] != 0 : false;
@@ -548,12 +561,12 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
     double rel;
     if (j == 1) {
         // cg.rs:80-84 initial test on ||r0||
-        const double bnorm = sqrt(fhd::reduce_partials256(bb_in, n_in, sh));
+        const double bnorm = sqrt(fhd::reduce_partials<CG_T>(bb_in, n_in, sh));
         const double r0 = sqrt(rr);
         rel = bnorm > 0.0 ? r0 / bnorm : r0;
         // true-residual modes: also stop at the FP64 rounding floor of x
         const double floor0 =
-            mode != M_REF ? 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh))
+            mode != M_REF ? 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials<CG_T>(fl_in, n_in, sh))
                           : 0.0;
         if (lead) {
             st->bnorm = bnorm;
@@ -585,7 +598,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
             // below tol, or at the floor that rounding x to FP64 alone leaves
             // (no FP64 vector does better), or close to that floor once
             // replacement cycles stop halving it (the attainable accuracy)
-            floorr = 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials256(fl_in, n_in, sh));
+            floorr = 2.0 * 1.1102230246251565e-16 * sqrt(fhd::reduce_partials<CG_T>(fl_in, n_in, sh));
             const double prev = st->r_rep[(nrep - 1) & 1];
             halved = res <= 0.5 * prev;
             nstag = halved ? 0u : st->n_stag[(nrep - 1) & 1] + 1u;
@@ -672,10 +685,10 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
     }
 
     const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
-    // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
-    const unsigned s = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
-    const unsigned lane = threadIdx.x & 63;
     double dot = 0.0;
+    // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
+    const unsigned s = __builtin_amdgcn_readfirstlane(blk * CG_SPB + (threadIdx.x >> 6));
+    const unsigned lane = threadIdx.x & 63;
     if (s < A.nslices) {
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
@@ -720,7 +733,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
             dot += xr[a] * acc[a];
         }
     }
-    dot = fhd::block_sum256(dot, sh);
+    dot = fhd::block_sum<CG_T>(dot, sh);
     if (threadIdx.x == 0) red_out[blockIdx.x] = dot;
 }
 
@@ -729,7 +742,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgMat A, const double* __restri
 // p_new = z + beta p_old is rebuilt from the (z, p) pairs exactly as the spmv
 // kernel formed it; the pairs are overwritten with (z_new, p_new).
 template <bool ORD, int PRE>
-__global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __restrict__ x,
+__global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __restrict__ x,
                                                    double* __restrict__ r,
                                                    const double* __restrict__ Ap,
                                                    double* __restrict__ zp,
@@ -737,11 +750,11 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
                                                    const double* __restrict__ red_in, int n_in,
                                                    double2* __restrict__ red_out) {
     typedef double pair_t __attribute__((ext_vector_type(2)));
-    __shared__ double sh[4];
+    __shared__ double sh[UPD_T / 64];
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_spmv;
     if (j != st->it_upd + 1) return;
-    const double pap = fhd::reduce_partials256(red_in, n_in, sh);
+    const double pap = fhd::reduce_partials<UPD_T>(red_in, n_in, sh);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (fabs(pap) < 1e-30) {  // cg.rs:102-107
         if (lead) {
@@ -754,9 +767,9 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
     const double beta = st->beta[j & 1];
     pair_t* zpv = reinterpret_cast<pair_t*>(zp);
     double rr = 0.0, rz = 0.0;
-    const uint32_t stride = gridDim.x * 256;
+    const uint32_t stride = gridDim.x * UPD_T;
     if (PRE == 0) {
-        for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < npad; i += stride) {
+        for (uint32_t i = blockIdx.x * UPD_T + threadIdx.x; i < npad; i += stride) {
             // cg.rs:112-115 with p of cg.rs:133-135, z of cg.rs:124-126
             const pair_t o = zpv[i];
             const double pi = madd(o.x, beta, o.y, ORD);
@@ -774,7 +787,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
         }
     } else {
         const uint32_t nn = npad / 3;
-        for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < nn; v += stride) {
+        for (uint32_t v = blockIdx.x * UPD_T + threadIdx.x; v < nn; v += stride) {
             const uint32_t s = v / SLICE, lane = v % SLICE;
             double rv[3], pv[3], zv[3];
 #pragma unroll
@@ -797,8 +810,8 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t npad, double* __rest
             }
         }
     }
-    rr = fhd::block_sum256(rr, sh);
-    rz = fhd::block_sum256(rz, sh);
+    rr = fhd::block_sum<UPD_T>(rr, sh);
+    rz = fhd::block_sum<UPD_T>(rz, sh);
     if (threadIdx.x == 0) red_out[blockIdx.x] = make_double2(rr, rz);
     if (lead) st->it_upd = j;
 }
@@ -914,8 +927,8 @@ __global__ __launch_bounds__(256) void k_true_res(CgMat A, const double* __restr
 
 static CgMat cgmat(const fh_matrix* a) {
     const Pattern& p = *a->pat;
-    return {p.slice_off.p, p.row_len.p, p.sell_cols.p, a->vals.p,     a->vals32.p,
-            a->tail64.p,     a->rowsum64.p,  a->rowsum32.p,   p.nrows, p.nslices};
+    return {p.slice_off.p, p.row_len.p, p.sell_cols.p, a->vals.p, a->vals32.p, a->tail64.p,
+            a->rowsum64.p, a->rowsum32.p, p.nrows, p.nslices};
 }
 
 static unsigned spmv_grid(const fh_matrix* a) { return std::max(1u, cdiv(a->pat->nslices, 4)); }
@@ -1042,7 +1055,8 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     const uint32_t n = a->n, nrows = a->pat->nrows, nsl = a->pat->nslices;
     const uint32_t npad = nsl * SLICE * BS;  // lane-interleaved vector length
     const unsigned gs = spmv_grid(a);
-    const unsigned gu = std::max(1u, std::min<unsigned>(cdiv(npad, 256 * 4), 1024));
+    const unsigned gu = std::max(1u, std::min<unsigned>(cdiv(npad, UPD_T * 4), UPD_MAX));
+    const unsigned gc = std::max(1u, cdiv(nsl, CG_SPB));  // k_cg_spmv blocks
     const auto t_wall0 = std::chrono::steady_clock::now();
     const uint32_t mode = args.precision == FH_PREC_FP64 ? M_REF
                         : args.precision == FH_PREC_FP64_TRUE ? M_TRUE
@@ -1082,7 +1096,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     else fill_f64(xl.p, npad, 0.0, s);
     fill_f64(zp.p, 2 * (size_t)npad, 0.0, s);
     double* x = xl.p;
-    DBuf<double> redA(dev, gs), bb(dev, gs), fl(dev, gs);
+    DBuf<double> redA(dev, gc), bb(dev, gs), fl(dev, gs);
     DBuf<double2> redB(dev, std::max(gs, gu));
 
     CgState* hs = host_state(dev);
@@ -1104,7 +1118,8 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, bl.p, x,
                        dinv.p, r.p, zp.p, redB.p, bb.p, fl.p);
     FH_LAUNCH_CHECK("residual");
-    int n_in_spmv = (int)gs;  // partial count feeding the next spmv kernel
+    // the next spmv follows k_residual (gs partials) rather than an update (gu)
+    bool after_res = true;
 
     const int M = std::max(1, env_int("FH_CG_CHUNK", 16));
     const bool use_graph = env_int("FH_CG_GRAPH", 1) != 0;
@@ -1123,20 +1138,20 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
     // the initial residual or a replacement, where the exact product restarts it
     const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
-    auto launch_chunk = [&](int nin_first, bool lowk) {
+    auto launch_chunk = [&](bool res_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
-            const int nin = m == 0 ? nin_first : (int)gu;
-            const bool rec = aprec && !(m == 0 && nin_first == (int)gs);
+            const int nin = (m == 0 && res_first) ? (int)gs : (int)gu;
+            const bool rec = aprec && !(m == 0 && res_first);
             hipEvent_t e0 = time_spmv ? ev[2 * m] : nullptr, e1 = time_spmv ? ev[2 * m + 1] : nullptr;
 #define FH_SPMV(D, L, R)                                                                         \
     do {                                                                                         \
         if (time_spmv)                                                                           \
-            hipExtLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, e0,  \
+            hipExtLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gc), dim3(CG_T), 0, s, e0, \
                                   e1, 0, A, (const double*)zp.p, Ap.p, st.p,                     \
                                   (const double2*)redB.p, nin, (const double*)bb.p,              \
                                   (const double*)fl.p, redA.p);                                  \
         else                                                                                     \
-            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gs), dim3(256), 0, s, A,      \
+            hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gc), dim3(CG_T), 0, s, A,     \
                                zp.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);               \
     } while (0)
             if constexpr (DIFF) {
@@ -1153,8 +1168,8 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                 FH_SPMV(false, false, false);
             }
 #undef FH_SPMV
-            hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(256), 0, s, npad, x, r.p,
-                               Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gs, redB.p);
+            hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
+                               r.p, Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gc, redB.p);
         }
     };
     uint64_t launched = 0;
@@ -1166,21 +1181,21 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         const uint64_t j0 = hs->it_upd;  // iterations done before this chunk
         // the first spmv after init / replacement reads gs residual partials;
         // later ones read gu update partials
-        if (use_graph && !time_spmv && !first_chunk && n_in_spmv == (int)gu) {
+        if (use_graph && !time_spmv && !first_chunk && !after_res) {
             const int g = lowk ? 1 : 0;
             if (!gexec2[g]) {
                 FH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                launch_chunk((int)gu, lowk);
+                launch_chunk(false, lowk);
                 FH_HIP(hipStreamEndCapture(s, &graph2[g]));
                 FH_HIP(hipGraphInstantiate(&gexec2[g], graph2[g], nullptr, nullptr, 0));
             }
             FH_HIP(hipGraphLaunch(gexec2[g], s));
         } else {
-            launch_chunk(n_in_spmv, lowk);
+            launch_chunk(after_res, lowk);
             FH_LAUNCH_CHECK("cg_chunk");
         }
         first_chunk = false;
-        n_in_spmv = (int)gu;
+        after_res = false;
         launched += M;
         FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
         FH_HIP(hipStreamSynchronize(s));
@@ -1215,7 +1230,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                                bl.p, x, dinv.p, r.p, zp.p, redB.p, (double*)nullptr, fl.p);
             hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, st.p);
             FH_LAUNCH_CHECK("replacement");
-            n_in_spmv = (int)gs;
+            after_res = true;
         }
         if (launched > args.max_iter + (hs->n_rep + 2) * (uint64_t)M + 64) break;  // safety net
     }

@@ -187,6 +187,41 @@ __device__ __forceinline__ double block_sum256(double v, double* sh) {
     return r;
 }
 
+// Deterministic block sum for NT threads (NT / 64 waves, a power of two);
+// result valid in every thread.  `sh` must hold NT / 64 doubles.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    constexpr int NW = NT / 64;
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) t[i] = sh[i];
+#pragma unroll
+    for (int n = NW; n > 1; n >>= 1)  // adjacent pairs: ((t0 + t1) + (t2 + t3)) + ...
+#pragma unroll
+        for (int i = 0; i < n / 2; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+    __syncthreads();
+    return t[0];
+}
+
+// Sum of m partials written by a previous kernel, in a fixed order, by an
+// NT-thread block; identical in every block that calls it.
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int m,
+                                                  double* sh) {
+    double s = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * NT < m; i += 4 * NT) {  // four independent loads in flight
+        const double a = part[i], b = part[i + NT], c = part[i + 2 * NT], d = part[i + 3 * NT];
+        s += (a + b) + (c + d);
+    }
+    for (; i < m; i += NT) s += part[i];
+    return block_sum<NT>(s, sh);
+}
+
 // Sum of m partials written by a previous kernel, in a fixed order, by a
 // 256-thread block; identical in every block that calls it.
 __device__ __forceinline__ double reduce_partials256(const double* __restrict__ part, int m,
