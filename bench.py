@@ -73,7 +73,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design (see DESIGN.md), not a measured path.",
+              "the multi-GPU sharding is designed but not a measured path.",
               file=sys.stderr)
         sys.exit(2)
 

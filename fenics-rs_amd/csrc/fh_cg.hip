@@ -950,7 +950,7 @@ static unsigned spmv_grid(const fh_matrix* a) { return std::max(1u, cdiv(a->pat-
 
 double spmv_algorithmic_bytes(const fh_matrix* a, int vbytes, int mbytes) {
     // matrix values + one 4-byte index per stored block + row pointers, one
-    // read of x and one write of y (SURVEY 8(d) CSR/BSR formula)
+    // read of x and one write of y (the CSR / BSR algorithmic byte count)
     const double nnzb = (double)a->pat->nnzb;
     const int bs2 = a->bs * a->bs;
     return nnzb * (bs2 * (double)mbytes + 4.0) + 4.0 * (a->pat->nrows + 1) +

@@ -5,7 +5,7 @@ multi-RHS solver.
 Tolerances: FP64 kernels that mirror the oracle are bitwise; the FP32 MFMA
 element matrices agree with the FP64 VALU ones to 2e-6 of the largest entry
 (FP32 rounding of the geometry and of 6-term sums); adjoint gradients agree
-with central finite differences to 1e-6 relative (north_star) and satisfy
+with central finite differences to 1e-6 relative (the required bar) and satisfy
 sum_c E_c dJ/dE_c = -J to 1e-9; batched FP64 columns agree with the oracle's
 CG to 1e-8 relative, mixed-precision ones to 1e-7."""
 import numpy as np
