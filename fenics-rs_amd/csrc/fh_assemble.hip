@@ -14,6 +14,7 @@
 //     bit-identical too.
 // Values are written straight into the SELL-64 layout the CG kernels read.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "fh_geom.hpp"
@@ -219,6 +220,54 @@ __global__ __launch_bounds__(256) void k_asm_p2_vec_mass(
             for (int al = 0; al < 3; ++al) vals[vidx(rc.soff, k, al * 3 + al, rc.lane, 9)] += mv;
         }
     }
+}
+
+// poisson_solver.rs:62-96 (triangles) / 135-172 (tetrahedra): per cell,
+// sum_q w_q |det J| (u_h(x_q) - u_exact(x_q))^2 with the degree-3 rule, the P1
+// basis at the reference points (1 - sum xi, xi...); block partial sums (the
+// host adds them in block order).  A degenerate cell raises *bad (the
+// reference's unwrap panics there).
+template <int D>
+__global__ __launch_bounds__(256) void k_l2_err(const double* __restrict__ V,
+                                                const uint32_t* __restrict__ C, uint32_t nc,
+                                                const double* __restrict__ u,
+                                                const double* __restrict__ uq, QuadTab q,
+                                                double* __restrict__ part, uint32_t* bad) {
+    __shared__ double sh[4];
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    double tot = 0.0;
+    if (c < nc) {
+        constexpr int K = D + 1;
+        double x[K][D];
+        uint32_t t[K];
+        load_cell<D, K>(V, C, c, x, t);
+        double detj;
+        bool ok;
+        if constexpr (D == 2) {
+            Map2 m;
+            ok = affine2(x[0], x[1], x[2], m);
+            detj = fabs(m.det);
+        } else {
+            Map3 m;
+            ok = affine3(x[0], x[1], x[2], x[3], m);
+            detj = fabs(m.det);
+        }
+        if (!ok) atomicOr(bad, 1u);
+        for (int k = 0; k < q.n; ++k) {
+            double phi[K];
+            phi[0] = D == 2 ? 1.0 - q.pts[k][0] - q.pts[k][1]
+                            : 1.0 - q.pts[k][0] - q.pts[k][1] - q.pts[k][2];
+#pragma unroll
+            for (int a = 1; a < K; ++a) phi[a] = q.pts[k][a - 1];
+            double ap = u[t[0]] * phi[0];
+#pragma unroll
+            for (int a = 1; a < K; ++a) ap = ap + u[t[a]] * phi[a];
+            const double err = ap - uq[(uint64_t)c * q.n + k];
+            tot += q.w[k] * detj * err * err;
+        }
+    }
+    tot = fhd::block_sum256(tot, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // Dirichlet fold (eigen.rs:51-78): the diagonal value of a constrained dof is
@@ -838,6 +887,38 @@ int fh_assemble_elasticity_mass_p2(uint64_t n_nodes, const double* nodes, uint64
         FH_CHECK(out, "null argument");
         *out = assemble_elasticity_mass_p2(resolve_device(device_idx), n_nodes, nodes, nc, cells,
                                            density);
+    });
+}
+
+int fh_l2_error(fh_mesh* m, const double* u, const double* u_exact_q, double* out) {
+    return guard([&] {
+        FH_CHECK(m && u && u_exact_q && out, "null argument");
+        DeviceCtx& d = device(m->dev);
+        hipStream_t s = d.stream;
+        const QuadTab q = quad_table(m->dim, 3);
+        const unsigned g = std::max(1u, cdiv(m->nc, 256));
+        DBuf<double> du(m->dev, std::max<uint32_t>(m->nv, 1)),
+            dq(m->dev, std::max<uint64_t>((uint64_t)m->nc * q.n, 1)), part(m->dev, g);
+        DBuf<uint32_t> bad(m->dev, 1);
+        fill_u32(bad.p, 1, 0, s);
+        h2d(du.p, u, m->nv, s);
+        h2d(dq.p, u_exact_q, (uint64_t)m->nc * q.n, s);
+        if (m->dim == 2)
+            hipLaunchKernelGGL(k_l2_err<2>, dim3(g), dim3(256), 0, s, m->verts.p, m->cells.p, m->nc,
+                               du.p, dq.p, q, part.p, bad.p);
+        else
+            hipLaunchKernelGGL(k_l2_err<3>, dim3(g), dim3(256), 0, s, m->verts.p, m->cells.p, m->nc,
+                               du.p, dq.p, q, part.p, bad.p);
+        FH_LAUNCH_CHECK("l2_err");
+        std::vector<double> h(g);
+        uint32_t hb = 0;
+        d2h(h.data(), part.p, g, s);
+        d2h(&hb, bad.p, 1, s);
+        FH_HIP(hipStreamSynchronize(s));
+        FH_CHECK(!hb, "degenerate cell in the L2 error integral");
+        double tot = 0.0;
+        for (double v : h) tot += v;
+        *out = std::sqrt(tot);
     });
 }
 

@@ -161,6 +161,7 @@ def lib():
         "fh_vibration_modes": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl, dbl, u64, C.c_int,
                                          vp, vp, vp, vp, C.POINTER(u64)]),
         "fh_matrix_from_csc": (C.c_int, [u64, vp, vp, vp, C.c_int, PP]),
+        "fh_l2_error": (C.c_int, [vp, vp, vp, C.POINTER(dbl)]),
         "fh_mesh_boundary_facets": (C.c_int, [vp, vp, vp, vp, C.POINTER(u64)]),
         "fh_boundary_points": (C.c_int, [vp, u64, vp, vp]),
         "fh_assemble_neumann_2d": (C.c_int, [vp, u64, vp, vp, vp]),
@@ -703,6 +704,20 @@ def solve_poisson_2d(mesh, source, boundary_value, tol, max_iter, precision=FP64
 
 
 solve_poisson_3d = solve_poisson_2d
+
+
+def compute_l2_error_2d(mesh: Mesh, u_h, u_exact) -> float:
+    """poisson_solver.rs:62-96 (and compute_l2_error_3d, :135-172): the L2
+    norm of u_h - u_exact with the degree-3 rule; u_exact(x: dim x N array)
+    -> values, or its values at mesh.quad_points(3)."""
+    uq = _sample(mesh, u_exact, degree=3)
+    u = _f64(u_h)
+    out = dbl()
+    _check(lib().fh_l2_error(mesh.h, _p(u), _p(uq), C.byref(out)))
+    return out.value
+
+
+compute_l2_error_3d = compute_l2_error_2d
 
 
 @dataclasses.dataclass

@@ -235,6 +235,10 @@ int fh_solve_elasticity_3d(fh_mesh* m, double E, double nu, const double* E_cell
 int fh_cell_bilinear_k0_dev(fh_mesh* m, double nu, const double* w_dev, const double* u_dev,
                             double* out_dev);
 int fh_cell_bilinear_k0(fh_mesh* m, double nu, const double* w, const double* u, double* out);
+/* replaces compute_l2_error_2d / compute_l2_error_3d (poisson_solver.rs:62-96,
+   135-172): u [nv] nodal values, u_exact_q [nc][nq] the exact solution at the
+   degree-3 points (fh_quad_points(m, 3, .)); a degenerate cell is an error */
+int fh_l2_error(fh_mesh* m, const double* u, const double* u_exact_q, double* out);
 /* elasticity.rs:230-320 compute_element_stresses: per cell stress[9],
  * strain[9], von_mises, trace (20 doubles)                                  */
 int fh_element_stresses(fh_mesh* m, const double* u, double E, double nu, double* out);

@@ -223,10 +223,13 @@ def test_c3_full_size_adjoint_vs_fd():
                      device="cuda:0", requires_grad=True)
     prob = ElasticityProblem(m, 0.3, [0, 0, -7.85e4], fh.clamp_dofs_x0(m.vertices), tol=1e-12,
                              precision=fh.FP64_TRUE)
+    import time
+    t0 = time.perf_counter()
     J = prob.compliance(E)
     J.backward()
     g = E.grad.detach().clone()
     Jv = float(J.detach())
+    print(f"C3 compliance + adjoint gradient: {(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
     assert abs(float((E.detach() * g).sum()) + Jv) < 1e-8 * abs(Jv)
     with torch.no_grad():
         u, _ = prob.solve(E.detach())
