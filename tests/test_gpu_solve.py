@@ -204,6 +204,31 @@ def test_error_paths():
         fh.ElasticMaterial(1.0, 0.5)
 
 
+def test_edge_cases():
+    # x0 already the solution: the initial test of cg.rs:80-84 returns at once
+    Ab, bb = c0_system()
+    cp, ri, va = Ab.export()
+    _, xo, _, _ = O.solve_cg(Ab, bb, None, 1e-12, 5000)
+    assert fh.solve_cg_gpu_or_cpu(cp, ri, va, bb, xo, 1e-6, 10).iterations == 0
+    # a 1x1 system
+    r = fh.solve_cg_gpu_or_cpu([0, 1], [0], [4.0], [2.0], None, 1e-12, 5)
+    assert abs(r.solution[0] - 0.5) < 1e-15 and r.iterations <= 1
+    # the smallest mesh: every vertex is on the boundary, u is the boundary data
+    m = fh.unit_square(1, 1)
+    sol = fh.solve_poisson_2d(m, src2d, lambda v: 1.0 + v[0], 1e-10, 10)
+    assert np.allclose(sol.u, 1.0 + m.vertices[:, 0], rtol=0, atol=1e-14)
+    # non-homogeneous boundary data on a larger mesh, against the oracle pipeline
+    gm, om = fh.unit_square(24, 17), O.unit_square(24, 17)
+    g = lambda v: np.sin(v[0]) + v[1] * v[1]  # noqa: E731
+    sol = fh.solve_poisson_2d(gm, src2d, g, 1e-12, 2000)
+    A = O.assemble_stiffness(om)
+    b = O.assemble_rhs(om, src2d(np.moveaxis(O.quad_points(om, 2), -1, 0)))
+    bd = O.boundary_dofs(om)
+    Ab2, bb2 = O.apply_dirichlet_bc(A, b, bd, np.array([g(om.vertices[int(i)]) for i in bd]))
+    _, xo2, _, _ = O.solve_cg(Ab2, bb2, None, 1e-12, 2000)
+    assert np.linalg.norm(sol.u - xo2) < 1e-9 * np.linalg.norm(xo2)
+
+
 def laplace7(nx):
     """gpu.rs:1614-1643: 3D 7-point finite-difference Laplacian."""
     n = nx ** 3
