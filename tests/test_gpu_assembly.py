@@ -188,3 +188,52 @@ def test_c2_full_size_p2_counts():
     assert K.nnz() == c["nnz_raw"]
     r = K.spmv(np.ones(nd))
     assert np.abs(r).max() < 1e-12
+
+
+def unstructured(om, seed, jitter):
+    """The oracle mesh with interior vertices moved by up to jitter * h,
+    vertices renumbered and cells permuted and rotated (orientation kept)."""
+    rng = np.random.default_rng(seed)
+    V = om.vertices.copy()
+    lo, hi = V.min(0), V.max(0)
+    interior = np.all((V > lo + 1e-9) & (V < hi - 1e-9), axis=1)
+    h = (hi - lo).min() / 8.0
+    V[interior] += jitter * h * rng.uniform(-1, 1, (int(interior.sum()), V.shape[1]))
+    perm = rng.permutation(om.nv)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(om.nv)
+    C = inv[om.cells.astype(np.int64)][rng.permutation(om.nc)]
+    k = C.shape[1]
+    C = np.array([np.roll(c, 2 * int(rng.integers(0, k // 2))) for c in C], dtype=np.uint64)
+    return O.Mesh(V[perm], C)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_unstructured_mesh_bitwise(dim):
+    """Assembly, loads, Dirichlet condensation and boundary dofs on a jittered,
+    renumbered mesh given through the C ABI: bitwise the oracle's."""
+    base = O.unit_square(9, 7) if dim == 2 else O.box_beam(1.0, 0.5, 0.25, 6, 4, 3)
+    om = unstructured(base, 7 + dim, 0.25)
+    gm = fh.Mesh.from_arrays(om.vertices, om.cells)
+    gpu_mesh_equals_oracle(gm, om)
+    same_csc(fh.assemble_stiffness(gm), O.assemble_stiffness(om))
+    if dim == 2:
+        same_csc(fh.assemble_mass(gm), O.assemble_mass_2d(om))
+    xo = O.quad_points(om, 2)
+    assert np.array_equal(bits(gm.quad_points(2)), bits(xo))
+    fq = np.cos(2 * xo[..., 0]) + xo[..., 1]
+    assert np.array_equal(bits(fh.assemble_rhs(gm, fq)), bits(O.assemble_rhs(om, fq)))
+    bd = O.boundary_dofs(om)
+    assert np.array_equal(gm.boundary_dofs(), bd)
+    if dim == 3:
+        K, Ko = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.33)), \
+            O.assemble_elasticity(om, 7e10, 0.33)
+        same_csc(K, Ko)
+        b, bo = fh.assemble_elasticity_body_force_3d(gm, [1.0, -2.0, 3.0]), \
+            O.assemble_body_force(om, [1.0, -2.0, 3.0])
+        assert np.array_equal(bits(b), bits(bo))
+        fixed = [(3 * int(i) + a, 0.01 * a) for i in bd[:9] for a in range(3)]
+        Kc, bc = fh.apply_vector_dirichlet_bc(K, b, fh.VectorDirichletBC(fixed))
+        Kco, bco = O.apply_dirichlet_bc(Ko, bo, [d for d, _ in fixed], [v for _, v in fixed])
+        same_csc(Kc, Kco)
+        assert np.array_equal(bits(bc), bits(bco))
