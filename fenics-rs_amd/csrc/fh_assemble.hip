@@ -1103,6 +1103,63 @@ int fh_matrix_export_csc(const fh_matrix* a, uint64_t* col_ptr, uint64_t* row_id
     });
 }
 
+int fh_matrix_export_bsr3(const fh_matrix* a, uint32_t* row_ptr, uint32_t* cols,
+                          double* vals_flat, double* dinv_flat) {
+    return guard([&] {
+        FH_CHECK(a && row_ptr, "null argument");
+        FH_CHECK(a->bs == 3, "matrix is not 3x3-blocked (assemble an elasticity operator)");
+        DeviceCtx& d = device(a->dev);
+        const Pattern& p = *a->pat;
+        std::vector<uint32_t> rp(p.nrows + 1);
+        d2h(rp.data(), p.row_ptr.p, rp.size(), d.stream);
+        FH_HIP(hipStreamSynchronize(d.stream));
+        std::copy(rp.begin(), rp.end(), row_ptr);
+        if (!cols && !vals_flat && !dinv_flat) return;
+        std::vector<uint32_t> cc(p.nnzb), soff(p.nslices + 1);
+        std::vector<double> vals(sell_value_count(a));
+        d2h(cc.data(), p.cols.p, cc.size(), d.stream);
+        d2h(soff.data(), p.slice_off.p, soff.size(), d.stream);
+        d2h(vals.data(), a->vals.p, vals.size(), d.stream);
+        FH_HIP(hipStreamSynchronize(d.stream));
+        if (cols) std::copy(cc.begin(), cc.end(), cols);
+        // Stored values are the operator itself: condensed couplings are
+        // explicit zeros, a constrained diagonal holds its value.
+        for (uint32_t r = 0; r < p.nrows; ++r) {
+            const uint32_t s = r / SLICE, l = r % SLICE;
+            bool have_diag = false;
+            for (uint32_t k = 0; k < rp[r + 1] - rp[r]; ++k) {
+                double blk[9];
+                for (int e = 0; e < 9; ++e)
+                    blk[e] = vals[vpos((uint64_t)soff[s] + (uint64_t)k * SLICE, e, l, 9)];
+                if (vals_flat) std::copy(blk, blk + 9, vals_flat + 9 * ((uint64_t)rp[r] + k));
+                if (dinv_flat && cc[rp[r] + k] == r) {
+                    // adjugate over determinant, cofactors in row-major order
+                    const double c00 = blk[4] * blk[8] - blk[5] * blk[7];
+                    const double c01 = blk[5] * blk[6] - blk[3] * blk[8];
+                    const double c02 = blk[3] * blk[7] - blk[4] * blk[6];
+                    const double det = blk[0] * c00 + blk[1] * c01 + blk[2] * c02;
+                    FH_CHECK(det != 0.0 && std::isfinite(det), "singular diagonal block at node " +
+                                                                   std::to_string(r));
+                    const double id = 1.0 / det;
+                    double* o = dinv_flat + 9 * (uint64_t)r;
+                    o[0] = c00 * id;
+                    o[1] = (blk[2] * blk[7] - blk[1] * blk[8]) * id;
+                    o[2] = (blk[1] * blk[5] - blk[2] * blk[4]) * id;
+                    o[3] = c01 * id;
+                    o[4] = (blk[0] * blk[8] - blk[2] * blk[6]) * id;
+                    o[5] = (blk[2] * blk[3] - blk[0] * blk[5]) * id;
+                    o[6] = c02 * id;
+                    o[7] = (blk[1] * blk[6] - blk[0] * blk[7]) * id;
+                    o[8] = (blk[0] * blk[4] - blk[1] * blk[3]) * id;
+                    have_diag = true;
+                }
+            }
+            FH_CHECK(!dinv_flat || have_diag, "node " + std::to_string(r) +
+                                                  " has no diagonal block");
+        }
+    });
+}
+
 int fh_matrix_destroy(fh_matrix* a) {
     return guard([&] {
         if (!a) return;

@@ -128,6 +128,7 @@ def lib():
         "fh_apply_dirichlet_dev": (C.c_int, [vp, vp, u64, vp, vp, PP, vp]),
         "fh_matrix_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_int)]),
         "fh_matrix_export_csc": (C.c_int, [vp, vp, vp, vp]),
+        "fh_matrix_export_bsr3": (C.c_int, [vp, vp, vp, vp, vp]),
         "fh_matrix_spmv": (C.c_int, [vp, vp, vp, C.c_int]),
         "fh_matrix_spmv_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "fh_matrix_destroy": (C.c_int, [vp]),
@@ -399,6 +400,20 @@ class SparseMatrix:
         va = np.empty(nnz)
         _check(lib().fh_matrix_export_csc(self.h, _p(cp), _p(ri), _p(va)))
         return cp, ri, va
+
+    def export_bsr3(self, with_dinv: bool = True):
+        """(n_nodes, row_ptr u32, cols u32, vals_flat f64[9 nnzb], dinv_flat f64[9 n_nodes]):
+        the arguments of GpuBsr3Context / solve_bsr3_pcg_gpu (gpu.rs:453-458,
+        775-786) for a 3x3-blocked operator; dinv_flat is None without with_dinv."""
+        n_nodes = self.nrows() // 3
+        rp = np.empty(n_nodes + 1, np.uint32)
+        _check(lib().fh_matrix_export_bsr3(self.h, _p(rp), None, None, None))
+        nnzb = int(rp[-1])
+        cc = np.empty(nnzb, np.uint32)
+        va = np.empty(9 * nnzb)
+        di = np.empty(9 * n_nodes) if with_dinv else None
+        _check(lib().fh_matrix_export_bsr3(self.h, _p(rp), _p(cc), _p(va), _p(di)))
+        return n_nodes, rp, cc, va, di
 
     def to_scipy(self):
         import scipy.sparse as sp

@@ -165,6 +165,16 @@ int fh_apply_dirichlet_dev(const fh_matrix* a, const double* b_in_dev, uint64_t 
 int fh_matrix_info(const fh_matrix* a, uint64_t* n, uint64_t* nnz, int* block);
 /* faer SparseColMat raw parts: col_ptr[n+1], row_idx[nnz], values[nnz]     */
 int fh_matrix_export_csc(const fh_matrix* a, uint64_t* col_ptr, uint64_t* row_idx, double* vals);
+/* The arrays GpuBsr3Context::new / solve_bsr3_pcg_gpu take (gpu.rs:453-458,
+ * 775-786) for a 3x3-blocked operator on any mesh: row_ptr[n_nodes+1],
+ * cols[nnzb] (ascending per row), vals_flat[9 nnzb] (row-major blocks,
+ * condensed couplings as explicit zeros) and dinv_flat[9 n_nodes], the
+ * inverse of each diagonal block (adjugate over determinant; a singular block
+ * is an error).  row_ptr is required; cols, vals_flat, dinv_flat may be null,
+ * so a first call can size the arrays from row_ptr[n_nodes].  The reference
+ * has no exporter: its callers build these arrays by hand.                  */
+int fh_matrix_export_bsr3(const fh_matrix* a, uint32_t* row_ptr, uint32_t* cols,
+                          double* vals_flat, double* dinv_flat);
 int fh_matrix_spmv(const fh_matrix* a, const double* x, double* y, int order);
 int fh_matrix_spmv_dev(const fh_matrix* a, const double* x_dev, double* y_dev, int order);
 int fh_matrix_destroy(fh_matrix* a);

@@ -311,3 +311,47 @@ def test_solver_determinism():
     a = fh.solve_poisson_2d(m, src2d, exact2d, 1e-12, 2000)
     b = fh.solve_poisson_2d(m, src2d, exact2d, 1e-12, 2000)
     assert a.iterations == b.iterations and np.array_equal(a.u, b.u)
+
+
+def test_export_bsr3_feeds_the_block_solver():  # gpu.rs:453-458, 775-786 arrays from any mesh
+    om = O.box_beam(1.0, 0.3, 0.2, 9, 3, 3)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(om.nv)  # renumbered vertices: not a generated layout
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(om.nv)
+    verts = om.vertices[perm]
+    cells = inv[om.cells][rng.permutation(len(om.cells))]
+    m = fh.Mesh.from_arrays(verts, cells)
+    # moduli near 1: the block solver's breakdown test is an absolute
+    # p.Ap <= 1e-30 (gpu.rs:1023), which a 1e-12 solve of a stiff, weakly loaded
+    # beam would reach
+    E_cell = 1.0 + rng.random(len(cells))
+    K = fh.assemble_elasticity_stiffness_3d(m, fh.ElasticMaterial(1.0, 0.3), E_cell)
+    b = fh.assemble_elasticity_body_force_3d(m, [0.0, 0.0, -100.0])
+    Kc, bc = fh.apply_vector_dirichlet_bc(K, b, fh.clamp_dofs_x0(verts))
+    n_nodes, rp, cc, va, di = Kc.export_bsr3()
+    assert n_nodes == om.nv and rp[0] == 0 and rp[-1] == len(cc)
+    for r in range(n_nodes):
+        row = cc[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0) and r in row
+    # the blocks hold exactly the exported CSC operator (itself bitwise the oracle's)
+    dense = np.zeros((3 * n_nodes, 3 * n_nodes))
+    for r in range(n_nodes):
+        for p in range(rp[r], rp[r + 1]):
+            c = int(cc[p])
+            dense[3 * r:3 * r + 3, 3 * c:3 * c + 3] = va[9 * p:9 * p + 9].reshape(3, 3)
+    assert np.array_equal(dense, Kc.to_scipy().toarray())
+    # dinv is the inverse of each diagonal block
+    D = np.stack([dense[3 * r:3 * r + 3, 3 * r:3 * r + 3] for r in range(n_nodes)])
+    Dinv = di.reshape(-1, 3, 3)
+    assert np.abs(Dinv @ D - np.eye(3)).max() < 1e-12
+    assert np.allclose(Dinv, np.linalg.inv(D), rtol=1e-12, atol=0)
+    # the exported arrays drive the block solver to the device CG's solution
+    res = fh.solve_bsr3_pcg_gpu(n_nodes, rp, cc, va, di, bc, None, 1e-12, 5000)
+    x, st = Kc.solve(bc, None, 1e-12, 5000)
+    assert np.linalg.norm(res.solution - x) / np.linalg.norm(x) < 1e-6  # cond(K) ~ 5e4
+    ctx = fh.GpuBsr3Context(n_nodes, rp, cc, va, di)
+    assert np.allclose(ctx.spmv(x), Kc.spmv(x), rtol=1e-13, atol=1e-9 * np.abs(bc).max())
+    # the scalar operator has no 3x3 blocks
+    with pytest.raises(fh.FenicsError, match="3x3"):
+        fh.assemble_stiffness_2d(fh.unit_square(3, 3)).export_bsr3()
