@@ -161,6 +161,8 @@ def lib():
         "fh_assemble_elasticity_mass_p2": (C.c_int, [u64, vp, u64, vp, dbl, C.c_int, PP]),
         "fh_vibration_modes": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl, dbl, u64, C.c_int,
                                          vp, vp, vp, vp, C.POINTER(u64)]),
+        "fh_vibration_modes_with_inner": (C.c_int, [vp, vp, u64, vp, u64, u64, dbl,
+                                                    InnerSolveFn, vp, vp, vp, vp, vp]),
         "fh_matrix_from_csc": (C.c_int, [u64, vp, vp, vp, C.c_int, PP]),
         "fh_l2_error": (C.c_int, [vp, vp, vp, C.POINTER(dbl)]),
         "fh_mesh_boundary_facets": (C.c_int, [vp, vp, vp, vp, C.POINTER(u64)]),
@@ -868,6 +870,48 @@ def solve_vibration_modes(k: SparseMatrix, m: SparseMatrix, fixed_dofs, num_mode
 
 class _ModeList(list):
     inner_iterations = 0
+
+
+InnerSolveFn = C.CFUNCTYPE(C.c_int, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.c_void_p)
+
+
+def solve_vibration_modes_with_inner(k: SparseMatrix, m: SparseMatrix, fixed_dofs,
+                                     num_modes: int, max_power_iters: int, tol: float, inner):
+    """eigen.rs:94-266 with the caller's inner solve: inner(b, x0) returns x
+    solving K x = b in the Dirichlet frame (K with constrained rows and columns
+    zero and a unit diagonal on them), x0 the previous iterate.  The power
+    iteration runs on the device; b, x0 and x cross to the host per call.  An
+    exception raised by inner is re-raised after the call."""
+    d = _u64(fixed_dofs)
+    n = k.n
+    f, om, ev = np.zeros(num_modes), np.zeros(num_modes), np.zeros(num_modes)
+    shapes = np.zeros((num_modes, n))
+    errors = []
+
+    def _inner(nn, bp, x0p, xp, _ctx):
+        try:
+            b = np.ctypeslib.as_array(bp, (nn,)).copy()
+            x0 = np.ctypeslib.as_array(x0p, (nn,)).copy()
+            x = np.asarray(inner(b, x0), dtype=np.float64)
+            if x.shape != (nn,):
+                raise FenicsError(FH_ERR_BAD_INPUT,
+                                  f"inner solve returned shape {x.shape}, expected ({nn},)")
+            np.ctypeslib.as_array(xp, (nn,))[:] = x
+            return 0
+        except BaseException as e:  # re-raised after the call
+            errors.append(e)
+            return 1
+
+    cb = InnerSolveFn(_inner)
+    code = lib().fh_vibration_modes_with_inner(k.h, m.h, len(d), _p(d), num_modes,
+                                               max_power_iters, tol, cb, None, _p(f), _p(om),
+                                               _p(ev), _p(shapes))
+    if errors:
+        raise errors[0]
+    _check(code)
+    return [VibrationMode(float(f[i]), float(om[i]), float(ev[i]), shapes[i])
+            for i in range(num_modes)]
 
 
 # ---------------------------------------------- boundary (ds) integrals

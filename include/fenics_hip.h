@@ -294,6 +294,22 @@ int fh_vibration_modes(const fh_matrix* K, const fh_matrix* M, uint64_t nfix,
                        double tol, double inner_tol, uint64_t inner_max_iter, int precision,
                        double* freq_hz, double* omega, double* eigenvalue, double* shapes,
                        uint64_t* inner_iterations);
+/* The caller's inner solve: x[n] solving K x = b[n] in the Dirichlet frame
+   (constrained rows and columns of K zero, unit diagonal on them), starting
+   from x0[n], the previous iterate.  Returns 0 on success; any other value
+   stops the iteration and the library returns FH_ERR_BAD_INPUT naming it. */
+typedef int (*fh_inner_solve_fn)(uint64_t n, const double* b, const double* x0, double* x,
+                                 void* ctx);
+/* replaces solve_vibration_modes_with_inner (eigen.rs:94-266) with the inner
+   closure (eigen.rs:101) as a function pointer plus context: the power
+   iteration, deflation and Rayleigh quotients run on the device; b and x0 are
+   copied to host memory for each call of `inner` and x back.  The library
+   calls `inner` on the calling thread, once per power iteration. */
+int fh_vibration_modes_with_inner(const fh_matrix* K, const fh_matrix* M, uint64_t nfix,
+                                  const uint64_t* fixed, uint64_t num_modes,
+                                  uint64_t max_power_iters, double tol, fh_inner_solve_fn inner,
+                                  void* inner_ctx, double* freq_hz, double* omega,
+                                  double* eigenvalue, double* shapes);
 
 /* ------------------------------- boundary facets and boundary (ds) integrals ------------------------------- */
 /* replaces extract_boundary_facets (mesh.rs:79-113 triangles, 145-210 tetrahedra):

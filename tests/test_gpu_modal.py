@@ -115,3 +115,55 @@ def test_inner_solve_failure_is_reported():
     with pytest.raises(fh.FenicsError) as e:
         fh.solve_vibration_modes(K, M, fixed, 1, 5, 1e-8, inner_tol=1e-14, inner_max_iter=3)
     assert e.value.code == fh.FH_ERR_NOT_CONVERGED
+
+
+def test_with_inner_device_cg_callback_is_bitwise_the_default():
+    """eigen.rs:282-300: solve_vibration_modes is solve_vibration_modes_with_inner
+    with solve_cg on the folded K; the device CG passed back as the caller's
+    inner solve (b and x0 through host memory) gives bitwise the same modes."""
+    gm, om, K, M, Ko, Mo, fixed = beam(8, 2, 2)
+    Kd = fh.fold_dirichlet(K, fixed, True)
+    calls = []
+
+    def inner(b, x0):
+        calls.append(len(b))
+        return Kd.solve(b, x0, 1e-9, 1000)[0]
+
+    a = fh.solve_vibration_modes(K, M, fixed, 2, 100, 1e-8)
+    b = fh.solve_vibration_modes_with_inner(K, M, fixed, 2, 100, 1e-8, inner)
+    assert calls and all(c == K.n for c in calls)
+    for ma, mb in zip(a, b):
+        assert bits(ma.eigenvalue) == bits(mb.eigenvalue)
+        assert np.array_equal(bits(ma.mode_shape), bits(mb.mode_shape))
+
+
+def test_with_inner_direct_solve_matches_oracle():
+    """A caller's sparse direct inner solve (eigen.rs:89-93: any solver of the
+    folded K) reaches the oracle's tightly converged eigenvalues."""
+    import scipy.sparse.linalg as spla
+    gm, om, K, M, Ko, Mo, fixed = beam(12, 2, 2)
+    lu = spla.splu(O.fold_dirichlet(Ko, fixed, True).to_scipy().tocsc())
+    modes = fh.solve_vibration_modes_with_inner(K, M, fixed, 3, 300, 1e-13,
+                                                lambda b, x0: lu.solve(b))
+    st, fo, omo, evo, sho = O.vibration_modes(Ko, Mo, fixed, 3, 300, 1e-13, inner_tol=1e-12,
+                                              inner_max_iter=5000)
+    assert st == 0
+    assert np.allclose([md.eigenvalue for md in modes], evo, rtol=1e-9, atol=0)
+    assert np.allclose([md.frequency_hz for md in modes], fo, rtol=1e-9, atol=0)
+    for md in modes:  # constrained entries stay at exactly zero
+        assert np.all(md.mode_shape[fixed.astype(np.int64)] == 0.0)
+
+
+def test_with_inner_errors_propagate():
+    """eigen.rs:189 propagates the inner closure's Err."""
+    gm, om, K, M, Ko, Mo, fixed = beam(4, 2, 2)
+
+    def boom(b, x0):
+        raise ValueError("inner solver exploded")
+
+    with pytest.raises(ValueError, match="exploded"):
+        fh.solve_vibration_modes_with_inner(K, M, fixed, 1, 10, 1e-8, boom)
+    with pytest.raises(fh.FenicsError, match="shape"):
+        fh.solve_vibration_modes_with_inner(K, M, fixed, 1, 10, 1e-8, lambda b, x0: b[:-1])
+    # the library stays usable afterwards
+    assert fh.solve_vibration_modes(K, M, fixed, 1, 50, 1e-6)[0].frequency_hz > 0
