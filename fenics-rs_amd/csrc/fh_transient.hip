@@ -101,7 +101,9 @@ fh_matrix* add_scaled(const fh_matrix* a, const fh_matrix* b, double alpha) {
     FH_CHECK(a->dev == b->dev, "add_scaled_matrices needs operands on one device");
     DeviceCtx& d = device(a->dev);
     hipStream_t s = d.stream;
-    if (!a->condensed && !b->condensed && same_layout(a, b, s)) {
+    // A condensed or folded operand exports fewer entries than it stores, so
+    // only plain assembled operators share the stored pattern with the result.
+    if (!a->condensed && !b->condensed && !a->folded && !b->folded && same_layout(a, b, s)) {
         std::unique_ptr<fh_matrix> o(new fh_matrix);
         o->dev = a->dev;
         o->bs = a->bs;

@@ -79,6 +79,14 @@ def test_csc_round_trip_and_general_union_bitwise():
                                   np.zeros(len(bnd)))
     M, Mo = fh.assemble_mass(gm), O.assemble_mass_2d(om)
     assert same_csc(fh.add_scaled_matrices(M, Kc, 0.1), O.add_scaled_matrices(Mo, Kco, 0.1))
+    # folded operands (eigen.rs:51-78 drops their exact zeros) share the stored
+    # pattern but not the exported one; entries zero in both must not appear
+    Kf, Mf = fh.fold_dirichlet(fh.assemble_stiffness(gm), bnd, True), fh.fold_dirichlet(M, bnd, False)
+    Kfo, Mfo = O.fold_dirichlet(O.assemble_stiffness(om), bnd, True), O.fold_dirichlet(Mo, bnd, False)
+    for alpha in (0.25, -1.0):
+        assert same_csc(fh.add_scaled_matrices(Mf, Kf, alpha),
+                        O.add_scaled_matrices(Mfo, Kfo, alpha))
+    assert same_csc(fh.add_scaled_matrices(M, Kf, 0.5), O.add_scaled_matrices(Mo, Kfo, 0.5))
 
 
 def test_step_exponential_decay_kat():  # transient.rs:104-126
