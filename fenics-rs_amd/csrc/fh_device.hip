@@ -207,6 +207,7 @@ const char* fh_last_error(void) { return g_last_error.c_str(); }
 
 int fh_device_count(int* count) {
     return guard([&] {
+        FH_CHECK(count, "null output");
         int c = 0;
         if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
         *count = c;
@@ -233,7 +234,10 @@ int fh_get_device_info(int index, fh_device_info* out) {
 }
 
 int fh_best_device(int* index) {
-    return guard([&] { *index = resolve_device(-1); });
+    return guard([&] {
+        FH_CHECK(index, "null output");
+        *index = resolve_device(-1);
+    });
 }
 
 int fh_synchronize(int dev) {

@@ -250,6 +250,7 @@ int fh_mesh_create(int dim, uint64_t nv, const double* vertices, uint64_t nc,
 
 int fh_mesh_unit_square(uint64_t nx, uint64_t ny, int device_idx, fh_mesh** out) {
     return guard([&] {
+        FH_CHECK(out, "null argument");
         FH_CHECK(nx > 0 && ny > 0, "nx and ny must be at least 1");
         const int dev = resolve_device(device_idx);
         DeviceCtx& d = device(dev);
@@ -266,6 +267,7 @@ int fh_mesh_unit_square(uint64_t nx, uint64_t ny, int device_idx, fh_mesh** out)
 int fh_mesh_box_beam(double L, double W, double H, uint64_t nx, uint64_t ny, uint64_t nz,
                      int device_idx, fh_mesh** out) {
     return guard([&] {
+        FH_CHECK(out, "null argument");
         FH_CHECK(nx > 0 && ny > 0 && nz > 0, "nx, ny, and nz must be at least 1");
         const int dev = resolve_device(device_idx);
         DeviceCtx& d = device(dev);

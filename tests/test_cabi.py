@@ -48,3 +48,32 @@ def test_oracle_is_not_linked_into_the_product():
     with open(fenics_hip.LIB_PATH, "rb") as f:
         blob = f.read()
     assert b"liboracle" not in blob and b"or_solve_cg" not in blob
+
+
+def test_null_arguments_are_rejected_not_dereferenced():
+    """Every entry point answers null pointers and zero sizes with a status and
+    a message (the reference's Err), never a crash; destroy calls accept null.
+    The checks run before any device work, so this needs no GPU."""
+    import fenics_hip as fh
+    L = fh.lib()
+    for name in declared_symbols():
+        if name == "fh_last_error":
+            continue
+        f = getattr(L, name)
+        args = []
+        for t in f.argtypes or []:
+            if isinstance(t, type) and issubclass(t, C._CFuncPtr):
+                args.append(C.cast(None, t))
+            elif t in (C.c_double, C.c_float):
+                args.append(0.0)
+            elif t in (C.c_int, C.c_uint64, C.c_uint32, C.c_size_t, C.c_int64):
+                args.append(0)
+            else:
+                args.append(None)
+        rc = f(*args)
+        if name.endswith("_destroy"):
+            assert rc == fh.FH_OK, name
+        elif name == "fh_synchronize":
+            assert rc in (fh.FH_OK, fh.FH_ERR_NO_DEVICE), name
+        else:
+            assert rc != fh.FH_OK and fh.last_error(), name
