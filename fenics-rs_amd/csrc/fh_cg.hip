@@ -480,28 +480,37 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 // value and column index in one 12-byte tail record loaded a step ahead: one
 // memory instruction fewer per block (measured -1.5% at C1; the FP32 and
 // scalar products measured 7-8% slower with tails, so they keep the form above).
+// The first step's loads (tail record and the eight paired values), which
+// depend on nothing the kernel computes: issued before the kernel's prologue.
+struct Tail3Head {
+    uint32_t base, w;  // slice start (entries) and width (steps)
+    uint32_t c;        // column of step 0
+    double vl;         // last value of step 0
+    double v[8];       // values 0..7 of step 0
+};
+
+__device__ __forceinline__ void tail3_head(const CgMat& A, uint32_t s, uint32_t lane,
+                                           Tail3Head& h) {
+    h.base = A.soff[s];
+    h.w = (A.soff[s + 1] - h.base) / SLICE;
+    if (h.w == 0) return;
+    ld_tail<true>(A.t64 + ((uint64_t)h.base + lane) * TAILW, h.vl, h.c);
+    const double* vk = A.v64 + (uint64_t)h.base * 9;
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) ld_pair<true>(vk + 128 * pp + 2 * lane, h.v[2 * pp], h.v[2 * pp + 1]);
+}
+
 template <bool DIFF, class G>
-__device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, uint32_t lane,
+__device__ __forceinline__ void row_product_tail3(const CgMat& A, const Tail3Head& h, uint32_t lane,
                                                   const double* xr, G&& gather, double* acc) {
-    const uint32_t base = A.soff[s], w = (A.soff[s + 1] - base) / SLICE;
+    const uint32_t base = h.base, w = h.w;
 #pragma unroll
     for (int a = 0; a < 3; ++a) acc[a] = 0.0;
     if (w == 0) return;
     const uint32_t* tp = A.t64 + ((uint64_t)base + lane) * TAILW;  // step k: + k * SLICE * TAILW
     const double* vb = A.v64 + (uint64_t)base * 9;              // step k: vb + k * SLICE * 9
     const uint32_t last = w - 1;
-    double vl;
-    uint32_t c;
-    ld_tail<true>(tp, vl, c);
-    for (uint32_t k = 0; k < w; ++k) {
-        double vn;
-        uint32_t cn;
-        ld_tail<true>(tp + (uint64_t)min(k + 1, last) * SLICE * TAILW, vn, cn);
-        double v[9];
-        const double* vk = vb + (uint64_t)k * SLICE * 9;
-#pragma unroll
-        for (int pp = 0; pp < 4; ++pp) ld_pair<true>(vk + 128 * pp + 2 * lane, v[2 * pp], v[2 * pp + 1]);
-        v[8] = vl;
+    auto step = [&](const double* v, uint32_t c) {
         double xg[3];
         gather(c, xg);
 #pragma unroll
@@ -511,6 +520,26 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, uint32_t s, ui
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int q = 0; q < 3; ++q) acc[a] = fma(v[a * 3 + q], xg[q], acc[a]);
+    };
+    double vl, vn;
+    uint32_t c, cn;
+    // step 0 from the head, the tail record of step 1 loaded ahead
+    ld_tail<true>(tp + (uint64_t)min(1u, last) * SLICE * TAILW, vl, c);
+    {
+        double v[9];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = h.v[e];
+        v[8] = h.vl;
+        step(v, h.c);
+    }
+    for (uint32_t k = 1; k < w; ++k) {
+        ld_tail<true>(tp + (uint64_t)min(k + 1, last) * SLICE * TAILW, vn, cn);
+        double v[9];
+        const double* vk = vb + (uint64_t)k * SLICE * 9;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) ld_pair<true>(vk + 128 * pp + 2 * lane, v[2 * pp], v[2 * pp + 1]);
+        v[8] = vl;
+        step(v, c);
         c = cn;
         vl = vn;
     }
@@ -562,6 +591,25 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
                                                  double* __restrict__ red_out) {
     static_assert(DIFF || !LOW, "the FP32 operator is only used in difference form");
     __shared__ double sh[CG_SPB];
+    // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
+    const unsigned xcd = xm.interleave ? 7 : blockIdx.x % 8;
+    const unsigned s = __builtin_amdgcn_readfirstlane(
+        xm.interleave ? blockIdx.x * CG_SPB + (threadIdx.x >> 6)
+                      : xm.b[xcd] + (blockIdx.x / 8) * CG_SPB + (threadIdx.x >> 6));
+    const unsigned lane = threadIdx.x & 63;
+    const bool active = s < xm.b[xcd + 1];
+    // Loads that depend on nothing computed below (the lane's own (z, p)
+    // pairs, the first stored block of its row) are issued before the state
+    // checks and the partial reduction, so their latency overlaps them.
+    constexpr bool HEAD = BS == 3 && !ORD && !LOW;
+    double xz[BS], po[BS];
+    Tail3Head head;
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < BS; ++q)
+            ld_pair<false>(zp + 2 * (((uint64_t)s * BS + q) * SLICE + lane), xz[q], po[q]);
+        if constexpr (HEAD) tail3_head(A, s, lane, head);
+    }
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
@@ -698,23 +746,12 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
     }
 
     double dot = 0.0;
-    // wave-uniform slice index (lets the compiler keep slice bounds in scalars)
-    const unsigned xcd = xm.interleave ? 7 : blockIdx.x % 8;
-    const unsigned s = __builtin_amdgcn_readfirstlane(
-        xm.interleave ? blockIdx.x * CG_SPB + (threadIdx.x >> 6)
-                      : xm.b[xcd] + (blockIdx.x / 8) * CG_SPB + (threadIdx.x >> 6));
-    const unsigned lane = threadIdx.x & 63;
-    if (s < xm.b[xcd + 1]) {
+    if (active) {
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        double xr[BS], xz[BS], acc[BS];
+        double xr[BS], acc[BS];
 #pragma unroll
-        for (int q = 0; q < BS; ++q) {
-            const uint64_t i = ((uint64_t)s * BS + q) * SLICE + lane;
-            double po;
-            ld_pair<false>(zp + 2 * i, xz[q], po);
-            xr[q] = madd(xz[q], beta, po, ORD);
-        }
+        for (int q = 0; q < BS; ++q) xr[q] = madd(xz[q], beta, po[q], ORD);
         auto gather = [&](uint32_t c, double* xv) {
 #pragma unroll
             for (int q = 0; q < BS; ++q) {
@@ -734,7 +771,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
         } else if constexpr (ORD) {
             row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
         } else if constexpr (BS == 3) {
-            row_product_tail3<DIFF>(A, s, lane, xo, gather, acc);
+            row_product_tail3<DIFF>(A, head, lane, xo, gather, acc);
         } else {
             row_product_pipe<BS, U, double, DIFF, true>(A, A.v64, s, lane, xo, gather, acc);
             if (DIFF) add_rowsum<BS>(A.S64, s, lane, xo, acc);
@@ -766,6 +803,29 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
                                                    double2* __restrict__ red_out) {
     typedef double pair_t __attribute__((ext_vector_type(2)));
     __shared__ double sh[UPD_T / 64];
+    pair_t* zpv = reinterpret_cast<pair_t*>(zp);
+    const uint64_t stride = (uint64_t)gridDim.x * UPD_T;
+    const uint64_t i0 = (uint64_t)blockIdx.x * UPD_T + threadIdx.x;
+    // The operands of the thread's first UPF elements do not depend on the
+    // reduced scalars: their loads are issued before the state checks and the
+    // partial reduction, so their latency overlaps that prologue.  The
+    // elements are then processed in the same order as by the plain loop.
+    constexpr int UPF = PRE == 0 ? 4 : 1;
+    pair_t f_zp[UPF];
+    double f_x[UPF], f_r[UPF], f_ap[UPF], f_d[UPF];
+    if constexpr (PRE == 0) {
+#pragma unroll
+        for (int u = 0; u < UPF; ++u) {
+            const uint64_t i = i0 + u * stride;
+            if (i < npad) {
+                f_zp[u] = zpv[i];
+                f_x[u] = x[i];
+                f_r[u] = r[i];
+                f_ap[u] = Ap[i];
+                f_d[u] = dinv[i];
+            }
+        }
+    }
     if (st->status != ST_RUNNING || st->need_rep) return;
     const uint64_t j = st->it_spmv;
     if (j != st->it_upd + 1) return;
@@ -780,17 +840,14 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
     }
     const double al = st->rz[j & 1] / pap;
     const double beta = st->beta[j & 1];
-    pair_t* zpv = reinterpret_cast<pair_t*>(zp);
     double rr = 0.0, rz = 0.0;
-    const uint32_t stride = gridDim.x * UPD_T;
     if (PRE == 0) {
-        for (uint32_t i = blockIdx.x * UPD_T + threadIdx.x; i < npad; i += stride) {
-            // cg.rs:112-115 with p of cg.rs:133-135, z of cg.rs:124-126
-            const pair_t o = zpv[i];
+        // cg.rs:112-115 with p of cg.rs:133-135, z of cg.rs:124-126
+        auto step = [&](uint64_t i, pair_t o, double xo, double ro, double ap, double di) {
             const double pi = madd(o.x, beta, o.y, ORD);
-            const double xi = ORD ? x[i] + al * pi : fma(al, pi, x[i]);
-            const double ri = ORD ? r[i] - al * Ap[i] : fma(-al, Ap[i], r[i]);
-            const double zi = ri * dinv[i];
+            const double xi = ORD ? xo + al * pi : fma(al, pi, xo);
+            const double ri = ORD ? ro - al * ap : fma(-al, ap, ro);
+            const double zi = ri * di;
             x[i] = xi;
             r[i] = ri;
             pair_t o2;
@@ -799,7 +856,14 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
             zpv[i] = o2;
             rr += ri * ri;
             rz += ri * zi;
+        };
+#pragma unroll
+        for (int u = 0; u < UPF; ++u) {
+            const uint64_t i = i0 + u * stride;
+            if (i < npad) step(i, f_zp[u], f_x[u], f_r[u], f_ap[u], f_d[u]);
         }
+        for (uint64_t i = i0 + UPF * stride; i < npad; i += stride)
+            step(i, zpv[i], x[i], r[i], Ap[i], dinv[i]);
     } else {
         const uint32_t nn = npad / 3;
         for (uint32_t v = blockIdx.x * UPD_T + threadIdx.x; v < nn; v += stride) {
