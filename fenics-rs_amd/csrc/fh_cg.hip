@@ -610,11 +610,14 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
             ld_pair<false>(zp + 2 * (((uint64_t)s * BS + q) * SLICE + lane), xz[q], po[q]);
         if constexpr (HEAD) tail3_head(A, s, lane, head);
     }
-    if (st->status != ST_RUNNING || st->need_rep) return;
+    // the state is read and the partials reduced before the early exit, so
+    // the two memory round trips overlap (a no-op launch reduces stale partials)
+    const uint32_t status = st->status, need_rep = st->need_rep;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
     const double tol = st->tol;
     const double2 rr_rz = reduce_pairs<CG_T>(red_in, n_in, sh);
+    if (status != ST_RUNNING || need_rep) return;
     const double rr = rr_rz.x, rz = rr_rz.y;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const bool low_prev = j > 1 ? st->low[(j - 1) & 1] != 0 : false;
@@ -826,10 +829,12 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
             }
         }
     }
-    if (st->status != ST_RUNNING || st->need_rep) return;
-    const uint64_t j = st->it_spmv;
-    if (j != st->it_upd + 1) return;
+    // state read and partials reduced before the early exits (as in k_cg_spmv)
+    const uint32_t status = st->status, need_rep = st->need_rep;
+    const uint64_t j = st->it_spmv, ju = st->it_upd;
     const double pap = fhd::reduce_partials<UPD_T>(red_in, n_in, sh);
+    if (status != ST_RUNNING || need_rep) return;
+    if (j != ju + 1) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (fabs(pap) < 1e-30) {  // cg.rs:102-107
         if (lead) {
