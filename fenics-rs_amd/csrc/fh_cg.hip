@@ -598,26 +598,33 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
                       : xm.b[xcd] + (blockIdx.x / 8) * CG_SPB + (threadIdx.x >> 6));
     const unsigned lane = threadIdx.x & 63;
     const bool active = s < xm.b[xcd + 1];
-    // Loads that depend on nothing computed below (the lane's own (z, p)
-    // pairs, the first stored block of its row) are issued before the state
-    // checks and the partial reduction, so their latency overlaps them.
-    constexpr bool HEAD = BS == 3 && !ORD && !LOW;
+    // FP64 3x3 product (EARLY): loads that depend on nothing computed below
+    // (the lane's own (z, p) pairs, the first stored block of its row) are
+    // issued before the state is read and the partials reduced, and the state
+    // is read before the reduction, so these memory round trips overlap.  A
+    // launch that turns out to be a no-op (the rest of a chunk after
+    // convergence or a replacement request) then pays for those loads: the
+    // other products keep the plain order, where no-ops are a larger share
+    // of the launches (short solves, scalar operators) and the gain smaller.
+    constexpr bool EARLY = BS == 3 && !ORD && !LOW;
     double xz[BS], po[BS];
     Tail3Head head;
-    if (active) {
+    auto own_pairs = [&] {
 #pragma unroll
         for (int q = 0; q < BS; ++q)
             ld_pair<false>(zp + 2 * (((uint64_t)s * BS + q) * SLICE + lane), xz[q], po[q]);
-        if constexpr (HEAD) tail3_head(A, s, lane, head);
+    };
+    if (EARLY && active) {
+        own_pairs();
+        tail3_head(A, s, lane, head);
     }
-    // the state is read and the partials reduced before the early exit, so
-    // the two memory round trips overlap (a no-op launch reduces stale partials)
     const uint32_t status = st->status, need_rep = st->need_rep;
+    if (!EARLY && (status != ST_RUNNING || need_rep)) return;
     const uint64_t j = st->it_upd + 1;
     const uint32_t mode = st->mode;
     const double tol = st->tol;
     const double2 rr_rz = reduce_pairs<CG_T>(red_in, n_in, sh);
-    if (status != ST_RUNNING || need_rep) return;
+    if (EARLY && (status != ST_RUNNING || need_rep)) return;
     const double rr = rr_rz.x, rz = rr_rz.y;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const bool low_prev = j > 1 ? st->low[(j - 1) & 1] != 0 : false;
@@ -753,6 +760,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
         const uint32_t row = s * SLICE + lane;
         const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
         double xr[BS], acc[BS];
+        if constexpr (!EARLY) own_pairs();
 #pragma unroll
         for (int q = 0; q < BS; ++q) xr[q] = madd(xz[q], beta, po[q], ORD);
         auto gather = [&](uint32_t c, double* xv) {
@@ -796,7 +804,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
 // PRE 1: 3x3 block Jacobi (gpu.rs bsr3_apply_dinv), one thread per block row.
 // p_new = z + beta p_old is rebuilt from the (z, p) pairs exactly as the spmv
 // kernel formed it; the pairs are overwritten with (z_new, p_new).
-template <bool ORD, int PRE>
+template <bool ORD, int PRE, bool EARLY>
 __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __restrict__ x,
                                                    double* __restrict__ r,
                                                    const double* __restrict__ Ap,
@@ -809,32 +817,28 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
     pair_t* zpv = reinterpret_cast<pair_t*>(zp);
     const uint64_t stride = (uint64_t)gridDim.x * UPD_T;
     const uint64_t i0 = (uint64_t)blockIdx.x * UPD_T + threadIdx.x;
-    // The operands of the thread's first UPF elements do not depend on the
-    // reduced scalars: their loads are issued before the state checks and the
-    // partial reduction, so their latency overlaps that prologue.  The
-    // elements are then processed in the same order as by the plain loop.
-    constexpr int UPF = PRE == 0 ? 4 : 1;
-    pair_t f_zp[UPF];
-    double f_x[UPF], f_r[UPF], f_ap[UPF], f_d[UPF];
-    if constexpr (PRE == 0) {
-#pragma unroll
-        for (int u = 0; u < UPF; ++u) {
-            const uint64_t i = i0 + u * stride;
-            if (i < npad) {
-                f_zp[u] = zpv[i];
-                f_x[u] = x[i];
-                f_r[u] = r[i];
-                f_ap[u] = Ap[i];
-                f_d[u] = dinv[i];
-            }
-        }
+    // EARLY (with the FP64 3x3 product, as in k_cg_spmv): the operands of the
+    // thread's first element do not depend on the reduced scalars, so their
+    // loads are issued before the state is read and the partials reduced, and
+    // the state is read before the reduction.  One element per thread covers
+    // that prologue; more only add to what a no-op launch streams.
+    constexpr bool PF = EARLY && PRE == 0;
+    pair_t f_zp = {0.0, 0.0};
+    double f_x = 0.0, f_r = 0.0, f_ap = 0.0, f_d = 0.0;
+    if (PF && i0 < npad) {
+        f_zp = zpv[i0];
+        f_x = x[i0];
+        f_r = r[i0];
+        f_ap = Ap[i0];
+        f_d = dinv[i0];
     }
-    // state read and partials reduced before the early exits (as in k_cg_spmv)
     const uint32_t status = st->status, need_rep = st->need_rep;
+    if (!EARLY && (status != ST_RUNNING || need_rep)) return;
     const uint64_t j = st->it_spmv, ju = st->it_upd;
+    if (!EARLY && j != ju + 1) return;
     const double pap = fhd::reduce_partials<UPD_T>(red_in, n_in, sh);
-    if (status != ST_RUNNING || need_rep) return;
-    if (j != ju + 1) return;
+    if (EARLY && (status != ST_RUNNING || need_rep)) return;
+    if (EARLY && j != ju + 1) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (fabs(pap) < 1e-30) {  // cg.rs:102-107
         if (lead) {
@@ -862,13 +866,12 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
             rr += ri * ri;
             rz += ri * zi;
         };
-#pragma unroll
-        for (int u = 0; u < UPF; ++u) {
-            const uint64_t i = i0 + u * stride;
-            if (i < npad) step(i, f_zp[u], f_x[u], f_r[u], f_ap[u], f_d[u]);
+        uint64_t i = i0;
+        if (PF) {
+            if (i < npad) step(i, f_zp, f_x, f_r, f_ap, f_d);
+            i += stride;
         }
-        for (uint64_t i = i0 + UPF * stride; i < npad; i += stride)
-            step(i, zpv[i], x[i], r[i], Ap[i], dinv[i]);
+        for (; i < npad; i += stride) step(i, zpv[i], x[i], r[i], Ap[i], dinv[i]);
     } else {
         const uint32_t nn = npad / 3;
         for (uint32_t v = blockIdx.x * UPD_T + threadIdx.x; v < nn; v += stride) {
@@ -1277,7 +1280,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                 FH_SPMV(false, false, false);
             }
 #undef FH_SPMV
-            hipLaunchKernelGGL((k_cg_update<ORD, PRE>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
+            hipLaunchKernelGGL((k_cg_update<ORD, PRE, BS == 3 && !ORD>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
                                r.p, Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gc, redB.p);
         }
     };
