@@ -77,3 +77,13 @@ def test_null_arguments_are_rejected_not_dereferenced():
             assert rc in (fh.FH_OK, fh.FH_ERR_NO_DEVICE), name
         else:
             assert rc != fh.FH_OK and fh.last_error(), name
+
+
+def test_rust_binding_stub_declares_the_header():
+    """The Rust extern block a maintainer copies from INTEGRATION.md names
+    exactly the entry points the header declares."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text[text.index('#[link(name = "fenics_hip")]'):]
+    block = block[:block.index("\n}\n")]
+    rust = sorted(set(re.findall(r"pub fn (fh_\w+)\(", block)))
+    assert rust == declared_symbols()
