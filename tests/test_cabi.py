@@ -80,8 +80,8 @@ def test_null_arguments_are_rejected_not_dereferenced():
 
 
 def test_rust_binding_stub_declares_the_header():
-    """The Rust extern block a maintainer copies from INTEGRATION.md names
-    exactly the entry points the header declares."""
+    """The Rust extern block of the integration guide (the binding a
+    maintainer copies) names exactly the entry points the header declares."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     block = text[text.index('#[link(name = "fenics_hip")]'):]
     block = block[:block.index("\n}\n")]
