@@ -64,7 +64,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=500,
+                    help="oracle CG iterations timed for the CPU baseline (about 10 s)")
     ap.add_argument("--nx", type=int, default=200, help="cantilever cells along x (ny=nz=nx/5)")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
