@@ -1102,7 +1102,8 @@ void ensure_tail(fh_matrix* a) {
     }
 }
 
-// pinned host mirror of the state, one per device
+// Pinned host mirror of the state, one per device (exported calls hold the
+// library's call lock, so one solve uses it at a time)
 static CgState* host_state(int dev) {
     static std::mutex mu;
     static std::map<int, CgState*> m;

@@ -42,10 +42,18 @@ void set_last_error(const std::string& msg);
 
 #define FH_LAUNCH_CHECK(name) FH_HIP(hipGetLastError())
 
+// One process-wide lock held for the whole of every exported call: calls
+// share each device's stream (solves capture HIP graphs on it) and the pinned
+// CG state mirror, so calls from several threads run one after the other.
+// Recursive, so an exported call may call another, or a caller's callback
+// may call back into the library on the same thread.
+std::recursive_mutex& api_lock();
+
 // Wrap the body of every exported function: never let an exception cross the
-// C ABI; record the message for fh_last_error().
+// C ABI; record the message for fh_last_error() (per thread).
 template <class F>
 int guard(F&& f) {
+    std::lock_guard<std::recursive_mutex> lk(api_lock());
     try {
         f();
         return FH_OK;

@@ -12,6 +12,10 @@ namespace fh {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
+std::recursive_mutex& api_lock() {
+    static std::recursive_mutex m;
+    return m;
+}
 
 static std::mutex g_dev_mu;
 static std::map<int, std::unique_ptr<DeviceCtx>> g_devs;

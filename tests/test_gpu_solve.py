@@ -355,3 +355,40 @@ def test_export_bsr3_feeds_the_block_solver():  # gpu.rs:453-458, 775-786 arrays
     # the scalar operator has no 3x3 blocks
     with pytest.raises(fh.FenicsError, match="3x3"):
         fh.assemble_stiffness_2d(fh.unit_square(3, 3)).export_bsr3()
+
+
+def test_calls_from_two_threads_match_sequential_calls():
+    """Calls from several threads share each device's stream and CG state, so
+    the library runs them one after the other: two threads assembling and
+    solving different systems get bitwise the results of a sequential run."""
+    import threading
+
+    def work(N):
+        m = fh.unit_square(N, N)
+        K = fh.assemble_stiffness(m)
+        bd = m.boundary_dofs()
+        b = np.ones(m.num_vertices()) / N ** 2
+        Kb, bb = fh.apply_dirichlet_bc(K, b, fh.DirichletBC(bd, np.zeros(len(bd))))
+        return [Kb.solve(bb, tol=1e-10, max_iter=5000, precision=prec)[0]
+                for prec in (fh.FP64, fh.FP64_TRUE, fh.MIXED)]
+
+    sizes = (40, 56)
+    seq = [work(N) for N in sizes]
+    out, errs = [None, None], []
+
+    def run(i):
+        try:
+            for _ in range(3):
+                out[i] = work(sizes[i])
+        except BaseException as e:  # reported after join
+            errs.append(e)
+
+    ts = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    for a, b in zip(seq, out):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
