@@ -445,12 +445,14 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_n
     const uint64_t nvec = V * NR;
     DBuf<double> f(dev, nvec), b(dev, nvec), x64(dev, nvec);
     {
-        std::vector<double> hf(nvec, 0.0);
+        // each instance's rows are contiguous on both sides: one copy per
+        // instance into its padded range, the padding rows zero
+        fill_f64(f.p, nvec, 0.0, s);
         uint64_t src = 0;
-        for (uint64_t i = 0; i < n_inst; ++i)
-            for (uint32_t r = 0; r < nvi[i]; ++r, ++src)
-                for (int k = 0; k < NR; ++k) hf[(uint64_t)(voff[i] + r) * NR + k] = f_nodal[src * NR + k];
-        h2d(f.p, hf.data(), nvec, s);
+        for (uint64_t i = 0; i < n_inst; ++i) {
+            h2d(f.p + (uint64_t)voff[i] * NR, f_nodal + src * NR, (size_t)nvi[i] * NR, s);
+            src += nvi[i];
+        }
         FH_HIP(hipStreamSynchronize(s));
     }
     const unsigned gslices = cdiv(Ac->pat->nslices, 4);
@@ -590,20 +592,22 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_n
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     // ---- outputs
-    std::vector<double> hx(nvec);
     std::vector<uint32_t> hit(ncol);
     std::vector<uint8_t> hd(ncol);
-    d2h(hx.data(), x64.p, nvec, s);
+    {
+        uint64_t dst = 0;
+        for (uint64_t i = 0; i < n_inst; ++i) {  // the caller's rows of each instance
+            d2h(u_out + dst * NR, x64.p + (uint64_t)voff[i] * NR, (size_t)nvi[i] * NR, s);
+            dst += nvi[i];
+        }
+    }
     d2h(hit.data(), diters.p, ncol, s);
     d2h(hd.data(), done.p, ncol, s);
     FH_HIP(hipStreamSynchronize(s));
-    uint64_t dst = 0;
     double dof_iters = 0.0;
     uint64_t max_it = 0;
     int code = FH_OK;
     for (uint64_t i = 0; i < n_inst; ++i) {
-        for (uint32_t rr = 0; rr < nvi[i]; ++rr, ++dst)
-            for (int k = 0; k < NR; ++k) u_out[dst * NR + k] = hx[(uint64_t)(voff[i] + rr) * NR + k];
         for (int k = 0; k < NR; ++k) {
             const uint64_t col = i * NR + k;
             if (iters_out) iters_out[col] = hit[col];
