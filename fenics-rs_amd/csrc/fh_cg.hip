@@ -480,6 +480,103 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 // value and column index in one 12-byte tail record loaded a step ahead: one
 // memory instruction fewer per block (measured -1.5% at C1; the FP32 and
 // scalar products measured 7-8% slower with tails, so they keep the form above).
+// FP32 3x3 product: the first step (column index and values) loaded before
+// the kernel's prologue, then the two-step pipeline from step 1 on.  Holding
+// one step across the prologue costs 10 VGPRs; two would cost 31.
+template <int BS, class TM>
+struct Pipe1Head {
+    uint32_t base, w, c;
+    TM v[BS * BS];
+};
+
+template <int BS, class TM, bool NT>
+__device__ __forceinline__ void pipe1_head(const CgMat& A, const TM* __restrict__ vals, uint32_t s,
+                                           uint32_t lane, Pipe1Head<BS, TM>& h) {
+    constexpr int B2 = BS * BS;
+    h.base = A.soff[s];
+    h.w = (A.soff[s + 1] - h.base) / SLICE;
+    if (h.w == 0) return;
+    h.c = ld_s<NT>(A.cols + (uint64_t)h.base + lane);
+    const TM* vk = vals + (uint64_t)h.base * B2;
+    if constexpr (B2 == 9) {
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) ld_pair<NT>(vk + 128 * pp + 2 * lane, h.v[2 * pp], h.v[2 * pp + 1]);
+        h.v[8] = ld_s<NT>(vk + 512 + lane);
+    } else {
+#pragma unroll
+        for (int j = 0; j < B2; ++j) h.v[j] = ld_s<NT>(vk + j * SLICE + lane);
+    }
+}
+
+template <int BS, int U, class TM, bool DIFF, bool NT, class G>
+__device__ __forceinline__ void row_product_pipe_h1(const CgMat& A, const TM* __restrict__ vals,
+                                                    const Pipe1Head<BS, TM>& h, uint32_t lane,
+                                                    const double* xr, G&& gather, double* acc) {
+    constexpr int B2 = BS * BS;
+    const uint32_t base = h.base, w = h.w;
+#pragma unroll
+    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
+    if (w == 0) return;
+    const uint32_t* cp = A.cols + (uint64_t)base + lane;
+    const TM* vb = vals + (uint64_t)base * B2;  // step k: vb + k * SLICE * B2, see vpos
+    const uint32_t last = w - 1;
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min(1u + u, last) * SLICE);
+    {
+        double xg[BS];
+        gather(h.c, xg);
+#pragma unroll
+        for (int q = 0; q < BS; ++q)
+            if (DIFF) xg[q] -= xr[q];
+#pragma unroll
+        for (int a = 0; a < BS; ++a)
+#pragma unroll
+            for (int q = 0; q < BS; ++q) acc[a] = fma((double)h.v[a * BS + q], xg[q], acc[a]);
+    }
+    for (uint32_t k = 1; k < w; k += U) {
+        uint32_t cn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            cn[u] = ld_s<NT>(cp + (uint64_t)min(k + U + u, last) * SLICE);
+        TM v[U][B2];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const TM* vk = vb + (uint64_t)min(k + u, last) * SLICE * B2;
+            if constexpr (B2 == 9) {
+#pragma unroll
+                for (int pp = 0; pp < 4; ++pp)
+                    ld_pair<NT>(vk + 128 * pp + 2 * lane, v[u][2 * pp], v[u][2 * pp + 1]);
+                v[u][8] = ld_s<NT>(vk + 512 + lane);
+            } else {
+#pragma unroll
+                for (int j = 0; j < B2; ++j) v[u][j] = ld_s<NT>(vk + j * SLICE + lane);
+            }
+        }
+        double xg[U][BS];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            gather(c[u], xg[u]);
+            // steps past the slice width re-read its last entry: add zero
+            const bool ok = u == 0 || k + u < w;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) {
+                const double d = DIFF ? xg[u][q] - xr[q] : xg[u][q];
+                xg[u][q] = ok ? d : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int a = 0; a < BS; ++a)
+#pragma unroll
+                for (int q = 0; q < BS; ++q)
+                    acc[a] = fma((double)v[u][a * BS + q], xg[u][q], acc[a]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = cn[u];
+    }
+}
+
 // The first step's loads (tail record and the eight paired values), which
 // depend on nothing the kernel computes: issued before the kernel's prologue.
 struct Tail3Head {
@@ -606,9 +703,12 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
     // convergence or a replacement request) then pays for those loads: the
     // other products keep the plain order, where no-ops are a larger share
     // of the launches (short solves, scalar operators) and the gain smaller.
-    constexpr bool EARLY = BS == 3 && !ORD && !LOW;
+    constexpr bool EARLY64 = BS == 3 && !ORD && !LOW;
+    constexpr bool EARLY32 = BS == 3 && LOW;  // the FP32 3x3 product: its first step
+    constexpr bool EARLY = EARLY64 || EARLY32;
     double xz[BS], po[BS];
     Tail3Head head;
+    Pipe1Head<BS, float> head32;
     auto own_pairs = [&] {
 #pragma unroll
         for (int q = 0; q < BS; ++q)
@@ -616,7 +716,8 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
     };
     if (EARLY && active) {
         own_pairs();
-        tail3_head(A, s, lane, head);
+        if constexpr (EARLY64) tail3_head(A, s, lane, head);
+        if constexpr (EARLY32) pipe1_head<BS, float, false>(A, A.v32, s, lane, head32);
     }
     const uint32_t status = st->status, need_rep = st->need_rep;
     if (!EARLY && (status != ST_RUNNING || need_rep)) return;
@@ -777,7 +878,10 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
         // measured 2-4% faster than two), two for FP32 ones, four scalars
         constexpr int U = BS == 1 ? 4 : (LOW ? 2 : 1);
         if constexpr (LOW) {
-            row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
+            if constexpr (EARLY32)
+                row_product_pipe_h1<BS, U, float, true, false>(A, A.v32, head32, lane, xo, gather, acc);
+            else
+                row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
             add_rowsum<BS>(A.S32, s, lane, xo, acc);
         } else if constexpr (ORD) {
             row_product<BS, double, true, DIFF>(A, A.v64, s, lane, len, xr, gather, acc);
