@@ -695,7 +695,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
                       : xm.b[xcd] + (blockIdx.x / 8) * CG_SPB + (threadIdx.x >> 6));
     const unsigned lane = threadIdx.x & 63;
     const bool active = s < xm.b[xcd + 1];
-    // FP64 3x3 product (EARLY): loads that depend on nothing computed below
+    // 3x3 products (EARLY): loads that depend on nothing computed below
     // (the lane's own (z, p) pairs, the first stored block of its row) are
     // issued before the state is read and the partials reduced, and the state
     // is read before the reduction, so these memory round trips overlap.  A
