@@ -90,6 +90,7 @@ struct CgState {
     uint64_t n_rep;
     uint64_t iters;
     uint64_t it_low;    // first iteration on the FP32 operator (0: none)
+    uint64_t n_low;     // iterations whose product read the FP32 values
     uint32_t want_low;  // relaxation threshold crossed: switch to the FP32 operator
     uint32_t stalled;   // FP32-operator cycle failed to reduce the true residual
     double rz[2];
@@ -853,6 +854,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
         st->low[j & 1] = LOW ? 1u : 0u;
         st->it_spmv = j;
         if (LOW && !low_prev) st->it_low = j;
+        if (LOW) st->n_low = st->n_low + 1;
         if (!LOW && mode == M_MIXED && rel < st->sw) st->want_low = 1;
     }
 
@@ -936,13 +938,19 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
         f_ap = Ap[i0];
         f_d = dinv[i0];
     }
+    // Whether this launch updates is decided on fields that only earlier
+    // kernels write (status, need_rep, it_spmv): every k_cg_spmv launch that
+    // does not advance it_spmv sets status or need_rep, so the two tests
+    // below are exactly "the preceding product was a full one".  it_upd, which
+    // block 0 of this kernel advances, is never read here: blocks need not be
+    // co-resident, and one that starts after block 0 has finished would see
+    // the new value.  The one field this kernel may write that it also reads,
+    // status (breakdown), is written only when every block decides the same.
     const uint32_t status = st->status, need_rep = st->need_rep;
     if (!EARLY && (status != ST_RUNNING || need_rep)) return;
-    const uint64_t j = st->it_spmv, ju = st->it_upd;
-    if (!EARLY && j != ju + 1) return;
+    const uint64_t j = st->it_spmv;
     const double pap = fhd::reduce_partials<UPD_T>(red_in, n_in, sh);
     if (EARLY && (status != ST_RUNNING || need_rep)) return;
-    if (EARLY && j != ju + 1) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (fabs(pap) < 1e-30) {  // cg.rs:102-107
         if (lead) {
@@ -1112,6 +1120,82 @@ __global__ __launch_bounds__(256) void k_true_res(CgMat A, const double* __restr
     tt = fhd::block_sum256(tt, sh);
     bsq = fhd::block_sum256(bsq, sh);
     if (threadIdx.x == 0) red[blockIdx.x] = make_double2(tt, bsq);
+}
+
+// t = b - A (xh + xl) in double-double for natural-layout [row][bs] vectors
+// (xl may be null: a plain FP64 vector), t rounded to FP64 into r (may be
+// null), block partials (t.t, b.b).  Every product a_ij xh_j and a_ij xl_j
+// enters the double-double sum with its rounding error (TwoProduct), so t is
+// accurate to about eps |t| + eps^2 sum_j |a_ij| |x_j|.
+template <int BS>
+__global__ __launch_bounds__(256) void k_residual_pair(CgMat A, const double* __restrict__ b,
+                                                       const double* __restrict__ xh,
+                                                       const double* __restrict__ xl,
+                                                       double* __restrict__ r,
+                                                       double2* __restrict__ red) {
+    __shared__ double sh[4];
+    const unsigned blk = fhd::xcd_block(blockIdx.x, gridDim.x);
+    const unsigned s = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    double tt = 0.0, bsq = 0.0;
+    if (s < A.nslices) {
+        const uint32_t row = s * SLICE + lane;
+        if (row < A.nrows) {
+            const uint32_t base = A.soff[s], len = A.rlen[row];
+            double hi[BS], lo[BS];
+#pragma unroll
+            for (int a = 0; a < BS; ++a) hi[a] = lo[a] = 0.0;
+            for (uint32_t k = 0; k < len; ++k) {
+                const uint64_t e = (uint64_t)base + (uint64_t)k * SLICE;
+                const uint32_t c = A.cols[e + lane];
+#pragma unroll
+                for (int q = 0; q < BS; ++q) {
+                    const double vh = xh[(uint64_t)c * BS + q];
+                    const double vl = xl ? xl[(uint64_t)c * BS + q] : 0.0;
+#pragma unroll
+                    for (int a = 0; a < BS; ++a) {
+                        const double av = A.v64[vpos(e, a * BS + q, lane, BS * BS)];
+                        dd_add_prod(hi[a], lo[a], av, vh);
+                        dd_add_prod(hi[a], lo[a], av, vl);
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                const double bi = b[(uint64_t)row * BS + a];
+                double d, e;
+                two_sum(bi, -hi[a], d, e);
+                const double t = d + (e - lo[a]);
+                if (r) r[(uint64_t)row * BS + a] = t;
+                tt += t * t;
+                bsq += bi * bi;
+            }
+        }
+    }
+    tt = fhd::block_sum256(tt, sh);
+    bsq = fhd::block_sum256(bsq, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = make_double2(tt, bsq);
+}
+
+// (hi, lo) += sc d elementwise (sc a power of two, so sc d is exact),
+// renormalised so that hi is the FP64 value nearest to hi + lo (TwoSum, then
+// Fast-TwoSum)
+__global__ void k_scale(double* __restrict__ v, double sc, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] *= sc;
+}
+
+__global__ void k_dd_add(double* __restrict__ hi, double* __restrict__ lo,
+                         const double* __restrict__ d, double sc, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        double s, e;
+        two_sum(hi[i], sc * d[i], s, e);
+        e += lo[i];
+        const double h = s + e;
+        lo[i] = e - (h - s);
+        hi[i] = h;
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -1485,7 +1569,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->iterations = hs->status == ST_RUNNING ? hs->it_upd : hs->iters;
-        stats->iterations_low = hs->it_low ? stats->iterations + 1 - hs->it_low : 0;
+        stats->iterations_low = hs->n_low;
         stats->replacements = hs->n_rep;
         stats->final_residual = hs->res;
         stats->rel_residual = hs->rel;
@@ -1531,11 +1615,118 @@ int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st) 
 #undef FH_RUN
 }
 
+// ||b - A (xh + xl)|| / ||b|| in double-double for natural-layout device
+// vectors (xl may be null); r (may be null) receives the residual rounded to
+// FP64.  Synchronises the stream.
+static void residual_pair(const fh_matrix* a, const double* b, const double* xh,
+                          const double* xl, double* r, hipStream_t s, double& rnorm,
+                          double& bnorm) {
+    const CgMat A = cgmat(a);
+    const unsigned g = spmv_grid(a);
+    DBuf<double2> red(a->dev, g);
+    if (a->bs == 1)
+        hipLaunchKernelGGL(k_residual_pair<1>, dim3(g), dim3(256), 0, s, A, b, xh, xl, r, red.p);
+    else
+        hipLaunchKernelGGL(k_residual_pair<3>, dim3(g), dim3(256), 0, s, A, b, xh, xl, r, red.p);
+    FH_LAUNCH_CHECK("residual_pair");
+    std::vector<double2> h(g);
+    d2h(h.data(), red.p, g, s);
+    FH_HIP(hipStreamSynchronize(s));
+    double tt = 0.0, bsq = 0.0;
+    for (const auto& v : h) {
+        tt += v.x;
+        bsq += v.y;
+    }
+    rnorm = std::sqrt(tt);
+    bnorm = std::sqrt(bsq);
+}
+
+double true_rel_residual_pair(const fh_matrix* a, const double* b, const double* xh,
+                              const double* xl, double* r, hipStream_t s) {
+    double rn, bn;
+    residual_pair(a, b, xh, xl, r, s, rn, bn);
+    return bn > 0.0 ? rn / bn : rn;
+}
+
+// Iterative refinement with the solution carried in double-double: r = b -
+// A (xh + xl) evaluated in double-double, A d = r solved by the FP64
+// true-residual CG to inner_tol, (xh, xl) += d, until the double-double
+// residual is below `target`.  xh is then hi + lo rounded to FP64, so its own
+// true residual is the residual of the FP64 vector nearest to the solution
+// found: a measured version of the floor that the FP64 solves estimate.
+static int refine_dd(const fh_matrix* a, const double* b, const double* x0, double target,
+                     uint64_t max_outer, double inner_tol, uint64_t max_iter, double* xh,
+                     double* xl, fh_refine_stats* out) {
+    DeviceCtx& d = device(a->dev);
+    hipStream_t s = d.stream;
+    const uint64_t n = a->n;
+    if (x0) d2d(xh, x0, n, s);
+    else fill_f64(xh, n, 0.0, s);
+    fill_f64(xl, n, 0.0, s);
+    DBuf<double> r(a->dev, n), dx(a->dev, n);
+    uint64_t k = 0, iters = 0;
+    double rn, bn;
+    residual_pair(a, b, xh, xl, r.p, s, rn, bn);
+    double rel = bn > 0.0 ? rn / bn : rn;
+    int code = FH_OK;
+    const unsigned g = std::max(1u, std::min(cdiv(n, 256), 4096u));
+    for (; k < max_outer && rel >= target && rn > 0.0; ++k) {
+        // the correction system is scaled to ||r|| ~ 1 by a power of two (exact):
+        // cg.rs's initial test on the absolute ||r0|| must not end it early
+        const double sc = std::ldexp(1.0, -std::ilogb(rn));
+        hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, s, r.p, sc, n);
+        SolveArgs sa{r.p, nullptr, dx.p, inner_tol, max_iter, FH_PREC_FP64_TRUE, FH_ORDER_FAST};
+        fh_solve_stats st{};
+        code = solve_device(a, sa, &st);
+        iters += st.iterations;
+        if (code == FH_ERR_BREAKDOWN) break;
+        hipLaunchKernelGGL(k_dd_add, dim3(g), dim3(256), 0, s, xh, xl, dx.p, 1.0 / sc, n);
+        FH_LAUNCH_CHECK("dd_add");
+        residual_pair(a, b, xh, xl, r.p, s, rn, bn);
+        rel = bn > 0.0 ? rn / bn : rn;
+    }
+    if (out) {
+        out->outer_steps = k;
+        out->cg_iterations = iters;
+        out->rel_residual_pair = rel;
+        out->rel_residual_rounded = true_rel_residual_pair(a, b, xh, nullptr, nullptr, s);
+    }
+    if (code == FH_ERR_BREAKDOWN) return code;
+    return rel < target ? FH_OK : FH_ERR_NOT_CONVERGED;
+}
+
 }  // namespace fh
 
 using namespace fh;
 
 extern "C" {
+
+int fh_true_residual_dev(const fh_matrix* a, const double* b_dev, const double* x_hi_dev,
+                         const double* x_lo_dev, double* rel_out) {
+    return guard([&] {
+        FH_CHECK(a && b_dev && x_hi_dev && rel_out, "null argument");
+        DeviceCtx& d = device(a->dev);
+        *rel_out = true_rel_residual_pair(a, b_dev, x_hi_dev, x_lo_dev, nullptr, d.stream);
+    });
+}
+
+int fh_solve_refined_dev(const fh_matrix* a, const double* b_dev, const double* x0_dev,
+                         double target, uint64_t max_outer, double inner_tol, uint64_t max_iter,
+                         double* x_hi_dev, double* x_lo_dev, fh_refine_stats* stats) {
+    int code = FH_OK;
+    const int g = guard([&] {
+        FH_CHECK(a && b_dev && x_hi_dev && x_lo_dev, "null argument");
+        FH_CHECK(target > 0.0 && inner_tol > 0.0 && max_outer > 0, "bad refinement parameters");
+        device(a->dev);
+        code = refine_dd(a, b_dev, x0_dev, target, max_outer, inner_tol, max_iter, x_hi_dev,
+                         x_lo_dev, stats);
+        if (code == FH_ERR_NOT_CONVERGED)
+            set_last_error("refinement did not reach the target residual");
+        else if (code == FH_ERR_BREAKDOWN)
+            set_last_error("Breakdown in CG: curvature p^T A p is near zero");
+    });
+    return g != FH_OK ? g : code;
+}
 
 int fh_matrix_spmv_dev(const fh_matrix* a, const double* x, double* y, int order) {
     return guard([&] {

@@ -75,6 +75,11 @@ class SolveStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class RefineStats(C.Structure):
+    _fields_ = [("outer_steps", C.c_uint64), ("cg_iterations", C.c_uint64),
+                ("rel_residual_pair", C.c_double), ("rel_residual_rounded", C.c_double)]
+
+
 _lib = None
 vp = C.c_void_p
 u64 = C.c_uint64
@@ -134,6 +139,9 @@ def lib():
         "fh_matrix_destroy": (C.c_int, [vp]),
         "fh_solve": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
         "fh_solve_dev": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
+        "fh_true_residual_dev": (C.c_int, [vp, vp, vp, vp, C.POINTER(dbl)]),
+        "fh_solve_refined_dev": (C.c_int, [vp, vp, vp, dbl, u64, dbl, u64, vp, vp,
+                                           C.POINTER(RefineStats)]),
         "fh_solve_csc_cg": (C.c_int, [u64, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
                                       C.POINTER(SolveStats)]),
         "fh_solve_csr_cg": (C.c_int, [u64, vp, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
@@ -195,6 +203,15 @@ def _check(code: int, allow=()):
     if code != FH_OK and code not in allow:
         raise FenicsError(code, last_error())
     return code
+
+
+def _dptr(t):
+    """Device pointer of a contiguous torch float64 tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous() or str(t.dtype) != "torch.float64" or not t.is_cuda:
+        raise FenicsError(FH_ERR_BAD_INPUT, "expected a contiguous float64 device tensor")
+    return C.c_void_p(t.data_ptr())
 
 
 def _p(a: Optional[np.ndarray]):
@@ -442,6 +459,36 @@ class SparseMatrix:
         elif code not in (FH_OK, FH_ERR_NOT_CONVERGED, FH_ERR_BREAKDOWN):
             _check(code)
         return x, st
+
+    def solve_dev(self, b_t, x_t, x0_t=None, tol=1e-10, max_iter=1000, precision=FP64,
+                  allow_fail=False):
+        """fh_solve_dev on torch float64 device tensors (x_t receives the
+        solution).  Returns SolveStats."""
+        st = SolveStats()
+        code = lib().fh_solve_dev(self.h, _dptr(b_t), _dptr(x0_t), tol, max_iter, precision,
+                                  _dptr(x_t), C.byref(st))
+        if not allow_fail or code not in (FH_OK, FH_ERR_NOT_CONVERGED, FH_ERR_BREAKDOWN):
+            _check(code)
+        return st
+
+    def true_residual_dev(self, b_t, xh_t, xl_t=None) -> float:
+        """||b - A (xh + xl)|| / ||b||, evaluated in double-double."""
+        out = C.c_double()
+        _check(lib().fh_true_residual_dev(self.h, _dptr(b_t), _dptr(xh_t), _dptr(xl_t),
+                                          C.byref(out)))
+        return out.value
+
+    def solve_refined_dev(self, b_t, xh_t, xl_t, x0_t=None, target=1e-14, max_outer=8,
+                          inner_tol=1e-6, max_iter=20000, allow_fail=False):
+        """Refinement with a double-double solution (xh_t + xl_t).  Returns
+        RefineStats; xh_t is the FP64 vector nearest to the solution."""
+        st = RefineStats()
+        code = lib().fh_solve_refined_dev(self.h, _dptr(b_t), _dptr(x0_t), target, max_outer,
+                                          inner_tol, max_iter, _dptr(xh_t), _dptr(xl_t),
+                                          C.byref(st))
+        if not allow_fail or code != FH_ERR_NOT_CONVERGED:
+            _check(code)
+        return st
 
 
 def _new_matrix(fn, *args) -> SparseMatrix:

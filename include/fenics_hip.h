@@ -180,15 +180,44 @@ int fh_matrix_spmv_dev(const fh_matrix* a, const double* x_dev, double* y_dev, i
 int fh_matrix_destroy(fh_matrix* a);
 
 /* ---------------------------------------------------------------- solve
- * cg.rs:40-149 solve_cg with Jacobi preconditioning (same stopping rule:
- * ||r_k||/||b|| < tol on the recursive residual; mixed modes additionally
- * require the FP64 true residual ||b - A x||/||b|| < tol).  x0 may be NULL.
- * Returns FH_ERR_NOT_CONVERGED / FH_ERR_BREAKDOWN like cg.rs (x_out is
- * still written).                                                            */
+ * cg.rs:40-149 solve_cg with Jacobi preconditioning.  FH_PREC_FP64 keeps the
+ * reference's stopping rule (||r_k||/||b|| < tol on the recursive residual).
+ * The true-residual modes (FH_PREC_FP64_TRUE, FH_PREC_MIXED, FH_PREC_MIXED_IR)
+ * stop on the double-double true residual ||b - A x||/||b||: when it is below
+ * tol, when it is at or below residual_floor (the residual that rounding x to
+ * FP64 alone leaves), or when replacement cycles stop halving it while it is
+ * within 8x of residual_floor; stats->true_rel_residual says which residual
+ * was reached.  Stagnation further above the floor returns
+ * FH_ERR_NOT_CONVERGED.  x0 may be NULL.  Returns FH_ERR_NOT_CONVERGED /
+ * FH_ERR_BREAKDOWN like cg.rs (x_out is still written).                      */
 int fh_solve(const fh_matrix* a, const double* b, const double* x0, double tol,
              uint64_t max_iter, int precision, double* x_out, fh_solve_stats* stats);
 int fh_solve_dev(const fh_matrix* a, const double* b_dev, const double* x0_dev, double tol,
                  uint64_t max_iter, int precision, double* x_dev, fh_solve_stats* stats);
+
+/* New (no counterpart in the reference, whose CG reports only its recursive
+ * residual, cg.rs:139-148): the true relative residual ||b - A x||/||b|| of a
+ * device vector x = x_hi + x_lo (x_lo may be NULL), evaluated in
+ * double-double.                                                           */
+int fh_true_residual_dev(const fh_matrix* a, const double* b_dev, const double* x_hi_dev,
+                         const double* x_lo_dev, double* rel_out);
+typedef struct fh_refine_stats {
+    uint64_t outer_steps;         /* correction solves                               */
+    uint64_t cg_iterations;       /* CG iterations over all correction solves        */
+    double rel_residual_pair;     /* ||b - A (x_hi + x_lo)|| / ||b||                  */
+    double rel_residual_rounded;  /* ||b - A x_hi|| / ||b||: x_hi is x_hi + x_lo
+                                     rounded to FP64, so this is the residual the
+                                     nearest FP64 vector to the solution leaves      */
+} fh_refine_stats;
+/* New (no counterpart in the reference): the solution in double-double.
+ * Iterative refinement from x0 (NULL: zero): r = b - A (x_hi + x_lo) in
+ * double-double, A d = r by the FH_PREC_FP64_TRUE CG to inner_tol (at most
+ * max_iter iterations), (x_hi, x_lo) += d, until rel_residual_pair < target
+ * or max_outer correction solves.  Returns FH_ERR_NOT_CONVERGED when the
+ * target is not reached (outputs still written).                           */
+int fh_solve_refined_dev(const fh_matrix* a, const double* b_dev, const double* x0_dev,
+                         double target, uint64_t max_outer, double inner_tol, uint64_t max_iter,
+                         double* x_hi_dev, double* x_lo_dev, fh_refine_stats* stats);
 
 /* ---------------------------------------------- drop-ins for gpu.rs entries */
 /* solve_cg (cg.rs:40-46) / solve_cg_gpu_or_cpu (gpu.rs:1542-1548) on faer's

@@ -577,3 +577,25 @@ def test_p2_vector_mass_integrates_quadratics():
     vol = 2.0 * 1.0 * 0.5
     exact = 7.0 * np.array([vol * 4 / 3, vol * 0.5 * 0.25, vol * (1 + 0.25 / 3)])
     assert np.allclose(tot, exact, rtol=1e-13, atol=0)
+
+
+def test_exact_residual_checker_vs_fractions():
+    """The exact-residual checker of the GPU residual tests agrees with a
+    rational-arithmetic evaluation on a small cantilever."""
+    from fractions import Fraction
+    from exactsum import exact_residual
+    om = O.box_beam(1.0, 0.2, 0.2, 4, 1, 1)
+    K = O.assemble_elasticity(om, 210e9, 0.3)
+    b = O.assemble_body_force(om, [0, 0, -7.85e4])
+    fx = O.clamp_dofs_x0(om)
+    Kb, bb = O.apply_dirichlet_bc(K, b, fx, np.zeros(len(fx)))
+    st, uo, it, res = O.solve_cg(Kb, bb, None, 1e-13, 2000)
+    cp, ri, va = Kb.export()
+    for x in (uo, np.random.default_rng(1).standard_normal(len(uo))):
+        r = [Fraction(v) for v in bb]
+        for c in range(len(cp) - 1):
+            for e in range(int(cp[c]), int(cp[c + 1])):
+                r[int(ri[e])] -= Fraction(va[e]) * Fraction(x[c])
+        rf = np.array([float(v) for v in r]).astype(np.longdouble)
+        ref = float(np.sqrt((rf ** 2).sum()) / np.sqrt((bb.astype(np.longdouble) ** 2).sum()))
+        assert exact_residual(Kb, bb, x) == ref
