@@ -1322,43 +1322,111 @@ static void to_li(const double* nat, double* out, uint32_t nrows, uint32_t nslic
     FH_LAUNCH_CHECK("to_li");
 }
 
-// The solve proper.
+// XCD ranges of k_cg_spmv for a pattern (computed once and kept on it; the
+// slice offsets are copied to the host for that): slice boundaries at each
+// eighth of the stored entries, or interleaved blocks when slice widths vary.
+static void xcd_map(const Pattern& pc, hipStream_t s, XcdMap& xm, unsigned& gc) {
+    Pattern& p = const_cast<Pattern&>(pc);
+    if (!p.xcd_ready) {
+        const uint32_t nsl = p.nslices;
+        std::vector<uint32_t> soff(nsl + 1);
+        d2h(soff.data(), p.slice_off.p, nsl + 1, s);
+        FH_HIP(hipStreamSynchronize(s));
+        const uint64_t tot = soff[nsl];
+        uint32_t per = 0;
+        p.xcd_b[0] = 0;
+        p.xcd_b[8] = nsl;
+        for (int xx = 1; xx < 8; ++xx) {
+            const uint64_t want = tot * (uint64_t)xx / 8;
+            p.xcd_b[xx] = (uint32_t)(std::lower_bound(soff.begin(), soff.end() - 1, (uint32_t)want) -
+                                     soff.begin());
+            p.xcd_b[xx] = std::max(p.xcd_b[xx], p.xcd_b[xx - 1]);
+        }
+        for (int xx = 0; xx < 8; ++xx) per = std::max(per, cdiv(p.xcd_b[xx + 1] - p.xcd_b[xx], CG_SPB));
+        p.xcd_blocks = 8 * std::max(per, 1u);
+        p.xcd_interleave = 0;
+        uint32_t wmax = 0;
+        for (uint32_t i = 0; i < nsl; ++i) wmax = std::max(wmax, soff[i + 1] - soff[i]);
+        if (nsl && (double)wmax > 1.5 * (double)tot / nsl) {
+            p.xcd_interleave = 1;  // b[8] = nsl bounds every block
+            p.xcd_blocks = std::max(1u, cdiv(nsl, CG_SPB));
+        }
+        p.xcd_ready = true;
+    }
+    for (int xx = 0; xx < 9; ++xx) xm.b[xx] = p.xcd_b[xx];
+    xm.interleave = p.xcd_interleave;
+    gc = p.xcd_blocks;
+}
+
+// Device buffers of one solve, carved from one workspace.
+struct CgWork {
+    double *r, *zp, *Ap, *xl, *bl, *xs0, *xs1, *dinv, *redA, *bb, *fl;
+    double2 *redB, *tr, *sum;
+    CgState* st;
+};
+
+template <class T>
+static T* carve(uint8_t* base, size_t& off, size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* q = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += std::max<size_t>(count, 1) * sizeof(T);
+    return q;
+}
+
+// Lays the buffers of a solve out from `base` (null: only sizes them);
+// returns the workspace bytes.  gc: k_cg_spmv blocks (xcd_map).
+static size_t cg_work(const fh_matrix* a, int pre, bool diff, unsigned gc, uint8_t* base,
+                      CgWork* w) {
+    const uint32_t nsl = a->pat->nslices;
+    const size_t npad = (size_t)nsl * SLICE * a->bs;
+    const unsigned gs = spmv_grid(a);
+    const unsigned gu = std::max(1u, std::min<unsigned>(cdiv(npad, UPD_T * 4), UPD_MAX));
+    size_t off = 0;
+    CgWork t{};
+    t.r = carve<double>(base, off, npad);
+    t.zp = carve<double>(base, off, 2 * npad);  // (z, p) pairs at each lane-interleaved index
+    t.Ap = carve<double>(base, off, npad);
+    t.xl = carve<double>(base, off, npad);
+    t.bl = carve<double>(base, off, npad);
+    if (diff) {  // the iterate at each of the last two replacements
+        t.xs0 = carve<double>(base, off, npad);
+        t.xs1 = carve<double>(base, off, npad);
+    }
+    t.dinv = carve<double>(base, off, pre == 0 ? npad : (size_t)nsl * SLICE * 9);
+    t.redA = carve<double>(base, off, gc);
+    t.bb = carve<double>(base, off, gs);
+    t.fl = carve<double>(base, off, gs);
+    t.redB = carve<double2>(base, off, std::max(gs, gu));
+    t.tr = carve<double2>(base, off, gs);
+    t.sum = carve<double2>(base, off, 1);
+    t.st = carve<CgState>(base, off, 1);
+    if (w) *w = t;
+    return (off + 255) & ~(size_t)255;
+}
+
+// fixed-order sum of m (a, b) partials into out[0] (one block)
+__global__ __launch_bounds__(256) void k_sum_pairs(const double2* __restrict__ p, int m,
+                                                   double2* __restrict__ out) {
+    __shared__ double sh[4];
+    const double2 v = reduce_pairs<256>(p, m, sh);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// The solve proper.  ws: the workspace (cg_work bytes) when the caller
+// brings one; the call then allocates nothing (no HIP graphs, no events, no
+// pinned host state), and the matrix must carry what its precision reads
+// (FP32 copy, row sums, tail records, XCD map), built by an earlier solve.
 template <int BS, bool ORD, bool DIFF, int PRE>
-static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* stats) {
+static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* stats,
+                  uint8_t* ws = nullptr) {
     const int dev = a->dev;
     DeviceCtx& d = device(dev);
     hipStream_t s = d.stream;
+    const bool noalloc = ws != nullptr;
     const uint32_t n = a->n, nrows = a->pat->nrows, nsl = a->pat->nslices;
     const uint32_t npad = nsl * SLICE * BS;  // lane-interleaved vector length
     const unsigned gs = spmv_grid(a);
     const unsigned gu = std::max(1u, std::min<unsigned>(cdiv(npad, UPD_T * 4), UPD_MAX));
-    // work-balanced XCD ranges: slice boundaries at each eighth of the entries
-    XcdMap xm{};
-    unsigned gc = 8;  // k_cg_spmv blocks
-    {
-        std::vector<uint32_t> soff(nsl + 1);
-        d2h(soff.data(), a->pat->slice_off.p, nsl + 1, s);
-        FH_HIP(hipStreamSynchronize(s));
-        const uint64_t tot = soff[nsl];
-        uint32_t per = 0;
-        xm.b[0] = 0;
-        xm.b[8] = nsl;
-        for (int xx = 1; xx < 8; ++xx) {
-            const uint64_t want = tot * (uint64_t)xx / 8;
-            xm.b[xx] = (uint32_t)(std::lower_bound(soff.begin(), soff.end() - 1, (uint32_t)want) -
-                                  soff.begin());
-            xm.b[xx] = std::max(xm.b[xx], xm.b[xx - 1]);
-        }
-        for (int xx = 0; xx < 8; ++xx) per = std::max(per, cdiv(xm.b[xx + 1] - xm.b[xx], CG_SPB));
-        gc = 8 * std::max(per, 1u);
-        uint32_t wmax = 0;
-        for (uint32_t i = 0; i < nsl; ++i) wmax = std::max(wmax, soff[i + 1] - soff[i]);
-        if (nsl && (double)wmax > 1.5 * (double)tot / nsl) {
-            xm.interleave = 1;  // b[8] = nsl bounds every block
-            gc = std::max(1u, cdiv(nsl, CG_SPB));
-        }
-    }
-    const auto t_wall0 = std::chrono::steady_clock::now();
     const uint32_t mode = args.precision == FH_PREC_FP64 ? M_REF
                         : args.precision == FH_PREC_FP64_TRUE ? M_TRUE
                         : args.precision == FH_PREC_MIXED ? M_MIXED
@@ -1370,37 +1438,48 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // sums; FH_CG_DIFF64=1 selects the difference form.
     const bool diff64 = DIFF && env_int("FH_CG_DIFF64", 0) != 0;
     fh_matrix* am = const_cast<fh_matrix*>(a);
-    if (uses32) ensure_vals32(am);
-    if (uses32 || diff64) ensure_rowsum(am, uses32);
-    if (!ORD && BS == 3) ensure_tail(am);
+    if (noalloc) {
+        FH_CHECK(a->pat->xcd_ready && (!uses32 || a->vals32.p) &&
+                     (!(uses32 || diff64) || a->rowsum64.p) && (!uses32 || a->rowsum32.p) &&
+                     (ORD || BS != 3 || a->tail64.p),
+                 "the allocation-free solve needs the matrix prepared for this precision: "
+                 "solve it once with fh_solve_dev (or fh_solve) in the same precision first");
+    } else {
+        if (uses32) ensure_vals32(am);
+        if (uses32 || diff64) ensure_rowsum(am, uses32);
+        if (!ORD && BS == 3) ensure_tail(am);
+    }
+    XcdMap xm{};
+    unsigned gc = 8;  // k_cg_spmv blocks
+    xcd_map(*a->pat, s, xm, gc);
+    const auto t_wall0 = std::chrono::steady_clock::now();
     const CgMat A = cgmat(a);
 
-    // zp: (z, p) pairs at each lane-interleaved index
-    DBuf<double> r(dev, npad), zp(dev, 2 * (size_t)npad), Ap(dev, npad);
-    DBuf<double> xl(dev, npad), bl(dev, npad);
-    // true-residual modes: the iterate at each of the last two replacements
-    DBuf<double> xsave[2];
-    if (DIFF) {
-        xsave[0].alloc(dev, npad);
-        xsave[1].alloc(dev, npad);
+    DBuf<uint8_t> own_ws;
+    if (!noalloc) {
+        own_ws.alloc(dev, cg_work(a, PRE, DIFF, gc, nullptr, nullptr));
+        ws = own_ws.p;
     }
-    DBuf<double> dinv(dev, PRE == 0 ? (size_t)npad : (size_t)nsl * SLICE * 9);
-    if (PRE == 1) to_li<9>(args.block_dinv, dinv.p, nrows, nsl, 0.0, s);
-    else if (args.inv_diag_override) to_li<BS>(args.inv_diag_override, dinv.p, nrows, nsl, 1.0, s);
+    CgWork w{};
+    cg_work(a, PRE, DIFF, gc, ws, &w);
+    double* const x = w.xl;
+    if (PRE == 1) to_li<9>(args.block_dinv, w.dinv, nrows, nsl, 0.0, s);
+    else if (args.inv_diag_override) to_li<BS>(args.inv_diag_override, w.dinv, nrows, nsl, 1.0, s);
     else {
         hipLaunchKernelGGL(k_inv_diag_li<BS>, dim3(std::max(1u, cdiv(npad, 256))), dim3(256), 0, s, A,
-                           dinv.p);
+                           w.dinv);
         FH_LAUNCH_CHECK("inv_diag_li");
     }
-    to_li<BS>(args.b, bl.p, nrows, nsl, 0.0, s);
-    if (args.x0) to_li<BS>(args.x0, xl.p, nrows, nsl, 0.0, s);
-    else fill_f64(xl.p, npad, 0.0, s);
-    fill_f64(zp.p, 2 * (size_t)npad, 0.0, s);
-    double* x = xl.p;
-    DBuf<double> redA(dev, gc), bb(dev, gs), fl(dev, gs);
-    DBuf<double2> redB(dev, std::max(gs, gu));
+    to_li<BS>(args.b, w.bl, nrows, nsl, 0.0, s);
+    if (args.x0) to_li<BS>(args.x0, x, nrows, nsl, 0.0, s);
+    else fill_f64(x, npad, 0.0, s);
+    fill_f64(w.zp, 2 * (size_t)npad, 0.0, s);
 
-    CgState* hs = host_state(dev);
+    // host mirror of the state: pinned, or (allocation-free calls) a static
+    // pageable copy per device
+    static CgState hs_static[64];
+    FH_CHECK(!noalloc || (dev >= 0 && dev < 64), "device index out of range");
+    CgState* hs = noalloc ? &hs_static[dev] : host_state(dev);
     std::memset(hs, 0, sizeof *hs);
     hs->tol = args.tol;
     hs->max_iter = args.max_iter;
@@ -1410,24 +1489,24 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     hs->mode = mode;
     hs->restart = mode == M_MIXED_IR ? 1 : 0;
     hs->rep_mark = 1;
-    DBuf<CgState> st(dev, 1);
-    FH_HIP(hipMemcpyAsync(st.p, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
+    FH_HIP(hipMemcpyAsync(w.st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
 
-    Timer tk(s);
+    std::unique_ptr<Timer> tk;
+    if (!noalloc) tk = std::make_unique<Timer>(s);
     // r0 = b - A x0 (FP64; double-double in the true-residual modes),
     // z0 = D^{-1} r0, partials (r.r, r.z), b.b
-    hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, bl.p, x,
-                       dinv.p, r.p, zp.p, redB.p, bb.p, fl.p);
+    hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, w.bl, x,
+                       w.dinv, w.r, w.zp, w.redB, w.bb, w.fl);
     FH_LAUNCH_CHECK("residual");
     // the next spmv follows k_residual (gs partials) rather than an update (gu)
     bool after_res = true;
 
     const int M = std::max(1, env_int("FH_CG_CHUNK", 16));
-    const bool use_graph = env_int("FH_CG_GRAPH", 1) != 0;
+    const bool use_graph = !noalloc && env_int("FH_CG_GRAPH", 1) != 0;
     // optional per-launch timing of the SpMV kernel: a start and a stop event
     // carried by the kernel's own dispatch (hipExtLaunchKernelGGL), so the pair
     // brackets its execution on this stream and nothing else
-    const bool time_spmv = env_int("FH_TIME_SPMV", 0) != 0;
+    const bool time_spmv = !noalloc && env_int("FH_TIME_SPMV", 0) != 0;
     const bool trace = env_int("FH_CG_TRACE", 0) != 0;  // per-chunk state on stderr
     std::vector<hipEvent_t> ev;
     double ms_hi = 0.0, ms_lo = 0.0;
@@ -1448,12 +1527,12 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     do {                                                                                         \
         if (time_spmv)                                                                           \
             hipExtLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gc), dim3(CG_T), 0, s, e0, \
-                                  e1, 0, A, xm, (const double*)zp.p, Ap.p, st.p,                 \
-                                  (const double2*)redB.p, nin, (const double*)bb.p,              \
-                                  (const double*)fl.p, redA.p);                                  \
+                                  e1, 0, A, xm, (const double*)w.zp, w.Ap, w.st,                 \
+                                  (const double2*)w.redB, nin, (const double*)w.bb,              \
+                                  (const double*)w.fl, w.redA);                                  \
         else                                                                                     \
             hipLaunchKernelGGL((k_cg_spmv<BS, ORD, D, L, R>), dim3(gc), dim3(CG_T), 0, s, A, xm, \
-                               zp.p, Ap.p, st.p, redB.p, nin, bb.p, fl.p, redA.p);               \
+                               w.zp, w.Ap, w.st, w.redB, nin, w.bb, w.fl, w.redA);               \
     } while (0)
             if constexpr (DIFF) {
                 if (rec) {
@@ -1470,7 +1549,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             }
 #undef FH_SPMV
             hipLaunchKernelGGL((k_cg_update<ORD, PRE, BS == 3 && !ORD>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
-                               r.p, Ap.p, zp.p, dinv.p, st.p, redA.p, (int)gc, redB.p);
+                               w.r, w.Ap, w.zp, w.dinv, w.st, w.redA, (int)gc, w.redB);
         }
     };
     uint64_t launched = 0;
@@ -1498,7 +1577,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         first_chunk = false;
         after_res = false;
         launched += M;
-        FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        FH_HIP(hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         FH_HIP(hipStreamSynchronize(s));
         if (time_spmv && hs->status == ST_RUNNING && !hs->need_rep && hs->it_upd == j0 + M) {
             // every SpMV of this chunk did a full product: attribute it by precision
@@ -1525,41 +1604,37 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                          (int)hs->stalled);
         if (hs->need_rep) {
             // keep the iterate this replacement evaluates (slot = count before k_mark)
-            d2d(xsave[hs->n_rep & 1].p, x, npad, s);
+            d2d((hs->n_rep & 1) ? w.xs1 : w.xs0, x, npad, s);
             // residual replacement: r = b - A x (double-double evaluation)
             hipLaunchKernelGGL((k_residual<BS, false, true, PRE>), dim3(gs), dim3(256), 0, s, A,
-                               bl.p, x, dinv.p, r.p, zp.p, redB.p, (double*)nullptr, fl.p);
-            hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, st.p);
+                               w.bl, x, w.dinv, w.r, w.zp, w.redB, (double*)nullptr, w.fl);
+            hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, w.st);
             FH_LAUNCH_CHECK("replacement");
             after_res = true;
         }
         if (launched > args.max_iter + (hs->n_rep + 2) * (uint64_t)M + 64) break;  // safety net
     }
-    const double kms = tk.stop_ms();
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int g = 0; g < 2; ++g) {
         if (gexec2[g]) (void)hipGraphExecDestroy(gexec2[g]);
         if (graph2[g]) (void)hipGraphDestroy(graph2[g]);
     }
 
-    FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
-    if (DIFF && hs->restore) d2d(x, xsave[hs->restore - 1].p, npad, s);
+    if (DIFF && hs->restore) d2d(x, hs->restore == 1 ? w.xs0 : w.xs1, npad, s);
     // the solution in the caller's layout, and its true residual (double-double)
     hipLaunchKernelGGL(k_from_li<BS>, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0, s, x, args.x,
                        nrows);
-    DBuf<double2> tr(dev, gs);
-    hipLaunchKernelGGL(k_true_res<BS>, dim3(gs), dim3(256), 0, s, A, bl.p, x, tr.p);
+    hipLaunchKernelGGL(k_true_res<BS>, dim3(gs), dim3(256), 0, s, A, w.bl, x, w.tr);
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, s, (const double2*)w.tr, (int)gs, w.sum);
     FH_LAUNCH_CHECK("true_res");
-    std::vector<double2> htr(gs);
-    d2h(htr.data(), tr.p, gs, s);
-    FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    double2 trs{0.0, 0.0};
+    FH_HIP(hipMemcpyAsync(&trs, w.sum, sizeof trs, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
-    double tt = 0, bsq = 0;
-    for (auto& v : htr) {
-        tt += v.x;
-        bsq += v.y;
-    }
+    const double kms = tk ? tk->stop_ms() : 0.0;
+    const double tt = trs.x, bsq = trs.y;
     const double wall =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
 
@@ -1576,7 +1651,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         stats->true_rel_residual = bsq > 0 ? std::sqrt(tt) / std::sqrt(bsq) : std::sqrt(tt);
         stats->residual_floor = hs->floor_rel;
         stats->wall_ms = wall;
-        stats->kernel_ms = kms;
+        stats->kernel_ms = tk ? kms : wall;
         stats->spmv_bytes = diff64 ? diff_spmv_bytes(a, 8) : spmv_algorithmic_bytes(a, 8, 8);
         stats->spmv32_bytes = diff_spmv_bytes(a, 4);
         // SpMV + the vector passes of the fused loop: the spmv kernel gathers
@@ -1585,9 +1660,10 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         const double dinv_b = PRE == 0 ? 8.0 * n : 24.0 * n;
         stats->iter_bytes = stats->spmv_bytes + 8.0 * n * (1.0 + 9.0) + dinv_b;
         const double it = (double)std::max<uint64_t>(stats->iterations, 1);
-        stats->effective_bw_gbps = stats->iter_bytes * it / (kms * 1e-3) / 1e9;
+        const double tms = std::max(stats->kernel_ms, 1e-9);
+        stats->effective_bw_gbps = stats->iter_bytes * it / (tms * 1e-3) / 1e9;
         const double flops = 2.0 * (double)a->nnz_scalar + 12.0 * n;
-        stats->gflops = flops * it / (kms * 1e-3) / 1e9;
+        stats->gflops = flops * it / (tms * 1e-3) / 1e9;
         stats->spmv_avg_ms = n_hi ? ms_hi / (double)n_hi : 0.0;
         stats->spmv_timed = n_hi;
         stats->spmv32_avg_ms = n_lo ? ms_lo / (double)n_lo : 0.0;
@@ -1599,20 +1675,29 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     return code;
 }
 
-int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st) {
+int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st, uint8_t* ws) {
     const bool ref = args.precision == FH_PREC_FP64;
     const bool ord = ref && args.order == FH_ORDER_ORDERED;
     const int pre = args.block_dinv ? 1 : 0;
     FH_CHECK(!(pre && a->bs != 3), "block Jacobi needs 3x3 blocks");
     FH_CHECK(args.precision >= 0 && args.precision <= 3, "unknown precision");
 #define FH_RUN(BS, PRE)                                                              \
-    return ref ? (ord ? run_cg<BS, true, false, PRE>(a, args, st)                   \
-                      : run_cg<BS, false, false, PRE>(a, args, st))                 \
-               : run_cg<BS, false, true, PRE>(a, args, st)
+    return ref ? (ord ? run_cg<BS, true, false, PRE>(a, args, st, ws)               \
+                      : run_cg<BS, false, false, PRE>(a, args, st, ws))             \
+               : run_cg<BS, false, true, PRE>(a, args, st, ws)
     if (a->bs == 1) FH_RUN(1, 0);
     if (pre) FH_RUN(3, 1);
     FH_RUN(3, 0);
 #undef FH_RUN
+}
+
+// workspace bytes of a solve of `a` (the allocation-free entry point's size check)
+size_t solve_workspace_bytes(const fh_matrix* a, const SolveArgs& args) {
+    XcdMap xm{};
+    unsigned gc = 8;
+    xcd_map(*a->pat, device(a->dev).stream, xm, gc);
+    const bool diff = args.precision != FH_PREC_FP64;
+    return cg_work(a, args.block_dinv ? 1 : 0, diff, gc, nullptr, nullptr);
 }
 
 // ||b - A (xh + xl)|| / ||b|| in double-double for natural-layout device
@@ -1695,11 +1780,67 @@ static int refine_dd(const fh_matrix* a, const double* b, const double* x0, doub
     return rel < target ? FH_OK : FH_ERR_NOT_CONVERGED;
 }
 
+// Scratch memory of the allocation-free solve (fh_solve_dev_noalloc): one
+// copy of this array per GPU, held until the process ends.  In host code
+// `scratch` is a host-side stand-in, not a device address: host code only
+// uses the address hipGetSymbolAddress returns for the current device
+// (scratch_ptr), in kernel arguments and copies.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)FH_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on device dev, looked up once per device with
+// that device current (the caller has made it so through device(dev)).
+// Exported calls hold the library's call lock, so the table needs no lock of
+// its own.
+static uint8_t* scratch_ptr(int dev) {
+    static uint8_t* ptr[64] = {};
+    FH_CHECK(dev >= 0 && dev < 64, "device index out of range");
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        FH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FH_CHECK(p != nullptr, "no device address for the solver scratch");
+        ptr[dev] = static_cast<uint8_t*>(p);
+    }
+    return ptr[dev];
+}
+
 }  // namespace fh
 
 using namespace fh;
 
 extern "C" {
+
+int fh_solve_dev_noalloc(const fh_matrix* a, const double* b_dev, const double* x0_dev,
+                         double tol, uint64_t max_iter, int precision, double* x_dev,
+                         fh_solve_stats* stats) {
+    int code = FH_OK;
+    const int g = guard([&] {
+        FH_CHECK(a && b_dev && x_dev, "null argument");
+        FH_CHECK(precision >= 0 && precision <= 3, "unknown precision");
+        device(a->dev);
+        uint8_t* ws = scratch_ptr(a->dev);
+        FH_CHECK(a->pat->xcd_ready,
+                 "the allocation-free solve needs the matrix prepared for this precision: "
+                 "solve it once with fh_solve_dev (or fh_solve) in the same precision first");
+        SolveArgs sa{b_dev, x0_dev, x_dev, tol, max_iter, precision,
+                     env_int("FH_ORDER", FH_ORDER_FAST)};
+        const size_t need = solve_workspace_bytes(a, sa);
+        if (need > NOALLOC_SCRATCH_BYTES) {
+            char m[256];
+            std::snprintf(m, sizeof m,
+                          "the allocation-free solve holds %d MiB of scratch per GPU; this "
+                          "system needs %.1f MiB (use fh_solve_dev)",
+                          (int)FH_NOALLOC_SCRATCH_MIB, need / 1048576.0);
+            throw Error(FH_ERR_BAD_INPUT, m);
+        }
+        code = solve_device(a, sa, stats, ws);
+        if (code == FH_ERR_NOT_CONVERGED)
+            set_last_error("CG failed to converge within the iteration limit");
+        else if (code == FH_ERR_BREAKDOWN)
+            set_last_error("Breakdown in CG: curvature p^T A p is near zero");
+    });
+    return g != FH_OK ? g : code;
+}
 
 int fh_true_residual_dev(const fh_matrix* a, const double* b_dev, const double* x_hi_dev,
                          const double* x_lo_dev, double* rel_out) {

@@ -36,6 +36,12 @@ struct Pattern {
     DBuf<uint32_t> sell_cols;   // [sell_entries]
     DBuf<uint32_t> row_len;     // [nrows]
     uint32_t max_width = 0;
+    // slice ranges of the 8 XCDs for the CG product, computed at the first
+    // solve on this pattern (fh_cg.hip xcd_map)
+    bool xcd_ready = false;
+    uint32_t xcd_b[9] = {};
+    uint32_t xcd_interleave = 0;
+    uint32_t xcd_blocks = 0;
 };
 
 // Dof -> cell adjacency of a cell-dof table (K dofs per cell), every dof's
@@ -163,7 +169,11 @@ struct SolveArgs {
     const double* inv_diag_override = nullptr;  // device, scalar Jacobi
     const double* block_dinv = nullptr;         // device, [nrows][9] block Jacobi
 };
-int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st);
+// ws: a caller-provided workspace of solve_workspace_bytes() (null: the
+// solve allocates its own)
+int solve_device(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* st,
+                 uint8_t* ws = nullptr);
+size_t solve_workspace_bytes(const fh_matrix* a, const SolveArgs& args);
 
 double spmv_algorithmic_bytes(const fh_matrix* a, int vbytes, int mbytes);
 

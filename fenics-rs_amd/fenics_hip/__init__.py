@@ -140,6 +140,8 @@ def lib():
         "fh_solve": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
         "fh_solve_dev": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp, C.POINTER(SolveStats)]),
         "fh_true_residual_dev": (C.c_int, [vp, vp, vp, vp, C.POINTER(dbl)]),
+        "fh_solve_dev_noalloc": (C.c_int, [vp, vp, vp, dbl, u64, C.c_int, vp,
+                                           C.POINTER(SolveStats)]),
         "fh_solve_refined_dev": (C.c_int, [vp, vp, vp, dbl, u64, dbl, u64, vp, vp,
                                            C.POINTER(RefineStats)]),
         "fh_solve_csc_cg": (C.c_int, [u64, vp, vp, vp, vp, vp, dbl, u64, C.c_int, vp,
@@ -467,6 +469,18 @@ class SparseMatrix:
         st = SolveStats()
         code = lib().fh_solve_dev(self.h, _dptr(b_t), _dptr(x0_t), tol, max_iter, precision,
                                   _dptr(x_t), C.byref(st))
+        if not allow_fail or code not in (FH_OK, FH_ERR_NOT_CONVERGED, FH_ERR_BREAKDOWN):
+            _check(code)
+        return st
+
+    def solve_dev_noalloc(self, b_t, x_t, x0_t=None, tol=1e-10, max_iter=1000, precision=FP64,
+                          allow_fail=False):
+        """fh_solve_dev_noalloc: as solve_dev, with every buffer in the
+        library's static per-GPU scratch (the matrix must have been solved once
+        with solve_dev / solve in the same precision).  Returns SolveStats."""
+        st = SolveStats()
+        code = lib().fh_solve_dev_noalloc(self.h, _dptr(b_t), _dptr(x0_t), tol, max_iter,
+                                          precision, _dptr(x_t), C.byref(st))
         if not allow_fail or code not in (FH_OK, FH_ERR_NOT_CONVERGED, FH_ERR_BREAKDOWN):
             _check(code)
         return st

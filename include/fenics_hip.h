@@ -195,6 +195,24 @@ int fh_solve(const fh_matrix* a, const double* b, const double* x0, double tol,
 int fh_solve_dev(const fh_matrix* a, const double* b_dev, const double* x0_dev, double tol,
                  uint64_t max_iter, int precision, double* x_dev, fh_solve_stats* stats);
 
+/* New (no counterpart in the reference), for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc):
+ * fh_solve_dev without any allocation.  Its buffers come from a static
+ * scratch array of FH_NOALLOC_SCRATCH_MIB MiB per GPU (held until the
+ * process ends), which bounds the system to about 1.8 million unknowns
+ * (FH_ERR_BAD_INPUT beyond; the message gives the size needed).  The matrix
+ * must already carry what its precision reads (FP32 copy, row sums, block
+ * tail records, XCD map): solve it once with fh_solve_dev or fh_solve in the
+ * same precision first, else FH_ERR_BAD_INPUT.  No HIP graphs, events or
+ * pinned memory: kernels are launched one by one, and kernel_ms reports the
+ * wall time.  Results are those of fh_solve_dev.  Every call on a GPU shares
+ * that GPU's scratch array, and each call leaves it holding its own state:
+ * the library's call lock serialises calls within a process, and work that
+ * a caller enqueues on the library's stream meanwhile must not touch it.   */
+#define FH_NOALLOC_SCRATCH_MIB 128
+int fh_solve_dev_noalloc(const fh_matrix* a, const double* b_dev, const double* x0_dev,
+                         double tol, uint64_t max_iter, int precision, double* x_dev,
+                         fh_solve_stats* stats);
 /* New (no counterpart in the reference, whose CG reports only its recursive
  * residual, cg.rs:139-148): the true relative residual ||b - A x||/||b|| of a
  * device vector x = x_hi + x_lo (x_lo may be NULL), evaluated in
