@@ -1,20 +1,33 @@
 #!/usr/bin/env python3
 """Headline benchmark: CG DOF*iter/s (+ SpMV HBM GB/s) of the mixed-precision
-GPU CG against the FP64 GPU CG at a 1e-10 true relative residual, on the 3D
-linear-elasticity P1 cantilever box_beam(10, 2, 2, 200, 40, 40) (1,013,643 DOF,
-1.92M tetrahedra, E = 210e9, nu = 0.3, clamped at x = 0, body force
+GPU CG against the FP64 GPU CG at equal accuracy, on the 3D linear-elasticity
+P1 cantilever box_beam(10, 2, 2, 200, 40, 40) (1,013,643 DOF, 1.92M
+tetrahedra, E = 210e9, nu = 0.3, clamped at x = 0, body force
 (0, 0, -7.85e4)).
 
 One step = the reference's solve_elasticity_3d hot path on device-resident
 inputs: numeric assembly of K and the body force, Dirichlet condensation,
 and Jacobi-PCG until the true residual ||b - A x|| / ||b|| (A x summed in
-double-double) is below 1e-10 or at the floor that rounding x to FP64 leaves
-(2-9e-10 on this system, where the reference's recursive-residual CG stops at
-a true 9.6e-9).  The symbolic pattern is built once per mesh in the warm-up
-(as the mesh itself, it depends only on the connectivity).  Prints one JSON
-line.
+double-double) is below 1e-10 or at the FP64 rounding floor.  On this system
+the floor is measured (outside the timed region) as the residual of the
+double-double solution rounded to FP64: 1.9e-10, so 1e-10 is out of reach for
+any FP64 vector near the solution, and the solves stop at about 2x that
+floor.  The symbolic pattern is built once per mesh in the warm-up (as the
+mesh itself, it depends only on the connectivity).  Prints one JSON line.
+
+Beside the timed steps (none of this is in `value`):
+  * FP64 GPU CG at equal accuracy, the reference's recursive-residual stop,
+    and classic FP32-inner iterative refinement (MIXED_IR);
+  * per-launch SpMV durations (HIP events on the kernel's own dispatch);
+  * the measured FP64 floor (double-double refinement);
+  * GPU time per phase (assembly, condensation, solve);
+  * device bandwidth ceilings (read stream, copy);
+  * P2 Poisson unit_cube(64^3) (2,146,689 DOF): SpMV and solve;
+  * the CPU oracle per phase: assembly over all host threads, the sequential
+    condensation and a bounded number of single-thread cg.rs iterations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--no-extra] [--ceiling]
 """
 import argparse
 import ctypes as C
@@ -35,7 +48,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profile_run(iters, mesh, E, nu, fbody, bc, u):
+def profile_run(iters, mesh, E, nu, fbody, bc):
     """A short, fixed amount of each SpMV kind for hardware-counter runs:
     FP64_TRUE uses the FP64-value kernel throughout, MIXED_IR the FP32-value
     one; both stop at `iters` iterations (not converged, by design)."""
@@ -58,15 +71,82 @@ def profile_run(iters, mesh, E, nu, fbody, bc, u):
     log(f"profile: 3 device copies of {nbytes} bytes")
 
 
+def ceilings(sizes_mib=(256, 1024, 4096), reps=5):
+    """Read-stream, copy and write bandwidth of the device (fh_stream_bandwidth)."""
+    import fenics_hip as fh
+    out = {}
+    for kind in ("read", "copy", "write"):
+        out[kind] = {}
+        for mib in sizes_mib:
+            g, ms = fh.stream_bandwidth(mib << 20, kind, reps)
+            out[kind][f"{mib}MiB"] = {"gbps": g, "ms": ms}
+            log(f"ceiling {kind} {mib} MiB: {g:.0f} GB/s ({ms * 1e3:.1f} us)")
+    return out
+
+
+def gbps(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9 if ms and ms > 0 else None
+
+
+def p2_poisson(fh, np, dev):
+    """C2: P2 Poisson unit_cube(64^3), f_c ~ U[0,1] per cell (seed 64), u = 0 on
+    the boundary; VALU FP64 and MFMA FP32 element paths, FP64_TRUE and MIXED
+    solves, per-launch SpMV times."""
+    import torch
+    gm = fh.unit_cube(64, 64, 64, device=dev)
+    nd, _ = gm.p2_info()
+    fc = np.random.default_rng(64).random(gm.nc)
+    b = fh.assemble_p2_rhs_cellconst(gm, fc)
+    bd = gm.p2_boundary_dofs()
+    out = {"workload": "P2 Poisson unit_cube(64,64,64), 2,146,689 DOF", "n_dof": nd}
+    K = fh.assemble_p2_stiffness(gm, 0)  # pattern built here (untimed)
+    del K
+    for method, name in ((0, "valu_fp64"), (1, "mfma_fp32")):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K = fh.assemble_p2_stiffness(gm, method)
+        out[f"assembly_{name}_ms"] = (time.perf_counter() - t0) * 1e3
+        del K
+    K = fh.assemble_p2_stiffness(gm, 2)
+    Kb, bb = fh.apply_dirichlet_bc(K, b, fh.DirichletBC(bd, np.zeros(len(bd))))
+    del K
+    bt = torch.tensor(bb, device=f"cuda:{dev}")
+    x = torch.empty_like(bt)
+    for prec, name in ((fh.FP64_TRUE, "fp64"), (fh.MIXED, "mixed")):
+        Kb.solve_dev(bt, x, tol=1e-10, max_iter=20000, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = Kb.solve_dev(bt, x, tol=1e-10, max_iter=20000, precision=prec)
+        ms = (time.perf_counter() - t0) * 1e3
+        out[name] = {"ms": ms, "iterations": int(st.iterations),
+                     "iterations_fp32_values": int(st.iterations_low),
+                     "true_rel_residual": st.true_rel_residual,
+                     "dof_iter_per_s": nd * int(st.iterations) / (ms * 1e-3)}
+    os.environ["FH_TIME_SPMV"] = "1"
+    st = Kb.solve_dev(bt, x, tol=1e-10, max_iter=20000, precision=fh.FP64_TRUE)
+    os.environ.pop("FH_TIME_SPMV", None)
+    ach = gbps(st.spmv_bytes, st.spmv_avg_ms)
+    out["spmv_fp64"] = {"avg_launch_us": st.spmv_avg_ms * 1e3, "bytes": st.spmv_bytes,
+                        "gbps": ach, "frac": ach / HBM_PEAK_GBS if ach else None,
+                        "launches": int(st.spmv_timed)}
+    log(f"C2: fp64 {out['fp64']['ms']:.1f} ms ({out['fp64']['iterations']} it), mixed "
+        f"{out['mixed']['ms']:.1f} ms; SpMV {st.spmv_avg_ms * 1e3:.1f} us = {ach:.0f} GB/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the side measurements (floor, MIXED_IR, phases, ceilings, P2)")
     ap.add_argument("--cpu-iters", type=int, default=500,
                     help="oracle CG iterations timed for the CPU baseline (about 10 s)")
     ap.add_argument("--nx", type=int, default=200, help="cantilever cells along x (ny=nz=nx/5)")
+    ap.add_argument("--ceiling", action="store_true",
+                    help="only measure the device bandwidth ceilings and print them as JSON")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
                          "and of the FP32-value SpMV on the cantilever, then a 1 GiB device "
@@ -88,6 +168,9 @@ def main():
         raise SystemExit("bench.py: no HIP device")
     log(f"device: {gpus[dev].name} ({gpus[dev].arch}), {gpus[dev].multiprocessor_count} CUs")
     torch.cuda.set_device(dev)
+    if args.ceiling:
+        print(json.dumps({"device": gpus[dev].name, "ceilings": ceilings()}))
+        return
 
     nx = args.nx
     ny = nz = nx // 5
@@ -103,19 +186,19 @@ def main():
     u_64 = torch.empty(n, dtype=torch.float64, device=f"cuda:{dev}")
     L = fh.lib()
 
-    def step(prec, u):
+    def step(prec, u, max_iter=50000, allow_fail=False):
         st = fh.SolveStats()
         code = L.fh_solve_elasticity_3d(mesh.h, E, nu, None, fbody.ctypes.data, len(dofs),
-                                        dofs.ctypes.data, vals.ctypes.data, tol, 50000, prec,
+                                        dofs.ctypes.data, vals.ctypes.data, tol, max_iter, prec,
                                         None, C.c_void_p(u.data_ptr()), C.byref(st))
-        if code != fh.FH_OK:
+        if code != fh.FH_OK and not (allow_fail and code == fh.FH_ERR_NOT_CONVERGED):
             raise fh.FenicsError(code, fh.last_error())
         return st
 
     log(f"mesh: {mesh.nv} vertices, {mesh.nc} cells, {n} DOF, {len(dofs)} fixed")
     os.environ.pop("FH_TIME_SPMV", None)
     if args.profile_iters > 0:
-        profile_run(args.profile_iters, mesh, E, nu, fbody, bc, u_64)
+        profile_run(args.profile_iters, mesh, E, nu, fbody, bc)
         return
     for w in range(args.warmup):
         st = step(fh.MIXED, u_mix)
@@ -139,12 +222,12 @@ def main():
     # ---------------- FP64 GPU CG at equal accuracy: FP64 SpMV, residual replaced
     # by the double-double true residual, same stopping rule as the mixed solve
     # (the reference's recursive-residual FP64 CG stops with a true residual
-    # ~10x larger on this system; it is reported beside it)
-    def timed(prec, u):
-        step(prec, u)
+    # ~20x larger on this system; it is reported beside it)
+    def timed(prec, u, **kw):
+        step(prec, u, **kw)
         torch.cuda.synchronize()
         ta = time.perf_counter()
-        st = step(prec, u)
+        st = step(prec, u, **kw)
         torch.cuda.synchronize()
         return st, (time.perf_counter() - ta) * 1e3
 
@@ -167,9 +250,6 @@ def main():
         f"({st64.spmv_bytes / 1e6:.1f} MB); mixed run: FP64 {sm.spmv_avg_ms * 1e3:.1f} us x "
         f"{sm.spmv_timed}, FP32 {sm.spmv32_avg_ms * 1e3:.1f} us x {sm.spmv32_timed} "
         f"({sm.spmv32_bytes / 1e6:.1f} MB)")
-
-    def gbps(nbytes, ms):
-        return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None
 
     # dominant kernel of the mixed solve by time: FP64-value or FP32-value SpMV
     t_hi = sm.spmv_avg_ms * sm.spmv_timed
@@ -198,6 +278,7 @@ def main():
         except (OSError, ValueError):
             pass
 
+    reached = max(s.true_rel_residual for s in stats)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF·iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -206,11 +287,12 @@ def main():
         "data": "synthetic (structured cantilever mesh generated on the GPU)",
         "config": {"workload": f"cantilever P1 elasticity box_beam(10,2,2,{nx},{ny},{nz}), "
                                f"{n} DOF, assembly + condensation + mixed-precision Jacobi-CG "
-                               f"to true rel. residual 1e-10 or the FP64 floor",
+                               f"to true rel. residual 1e-10 or the FP64 floor "
+                               f"(reached {reached:.2e})",
                    "n_dof": n, "cells": mesh.nc, "tol": tol},
         "iterations": total_iters / args.steps,
         "iterations_fp32_values": float(np.mean([s.iterations_low for s in stats])),
-        "true_rel_residual": max(s.true_rel_residual for s in stats),
+        "true_rel_residual": reached,
         "residual_floor": stats[-1].residual_floor,
         "rel_diff_vs_fp64": rel_err,
         "fp64": {"value": n * int(s64.iterations) / (ms64 * 1e-3), "ms_per_step": ms64,
@@ -225,27 +307,90 @@ def main():
         "cpu_baseline": None,
     }
 
-    # ---------------- CPU baseline: the oracle's single-thread cg.rs loop on this matrix
+    if not args.no_extra:
+        # ---------------- classic iterative refinement: FP32 values from the first
+        # iteration, recurrences restarted at each FP64 replacement (capped)
+        cap = 4 * int(s64.iterations)
+        u_ir = torch.empty_like(u_64)
+        sir, msir = timed(fh.MIXED_IR, u_ir, max_iter=cap, allow_fail=True)
+        out["mixed_ir"] = {
+            "ms_per_step": msir, "iterations": int(sir.iterations), "iteration_cap": cap,
+            "converged": sir.status == fh.FH_OK, "true_rel_residual": sir.true_rel_residual,
+            "replacements": int(sir.replacements),
+            "mode": "FP32 matrix values from the start, restart at every replacement"}
+        log(f"mixed_ir: {sir.iterations} it (cap {cap}), {msir:.1f} ms, true rel res "
+            f"{sir.true_rel_residual:.2e}, status {sir.status}")
+
+        # ---------------- phases on the GPU and the measured FP64 floor
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        K = fh.assemble_elasticity_stiffness_3d(mesh, fh.ElasticMaterial(E, nu))
+        b = fh.assemble_elasticity_body_force_3d(mesh, fbody)
+        tb = time.perf_counter()
+        Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
+        tcd = time.perf_counter()
+        del K
+        bt = torch.tensor(bb, device=f"cuda:{dev}")
+        x = torch.empty_like(bt)
+        Kb.solve_dev(bt, x, tol=tol, max_iter=50000, precision=fh.MIXED)
+        torch.cuda.synchronize()
+        td = time.perf_counter()
+        out["gpu_phases_ms"] = {"assembly_and_body_force": (tb - ta) * 1e3,
+                                "condensation": (tcd - tb) * 1e3, "solve": (td - tcd) * 1e3,
+                                "note": "host wall time per call; assembly and condensation "
+                                        "include the host copies of b"}
+        xh, xl = torch.empty_like(bt), torch.empty_like(bt)
+        rs = Kb.solve_refined_dev(bt, xh, xl, target=1e-14, max_outer=8, inner_tol=1e-6,
+                                  max_iter=50000, allow_fail=True)
+        out["residual_floor_measured"] = rs.rel_residual_rounded
+        out["residual_floor_measured_note"] = (
+            "true residual of the double-double solution (residual "
+            f"{rs.rel_residual_pair:.1e}) rounded to FP64: no FP64 vector near the solution "
+            "goes much below it")
+        out["true_rel_residual_over_measured_floor"] = reached / rs.rel_residual_rounded
+        log(f"measured floor {rs.rel_residual_rounded:.3e} (pair residual "
+            f"{rs.rel_residual_pair:.2e}, {rs.cg_iterations} CG iterations)")
+        del Kb, xh, xl, x, bt
+
+        out["ceilings"] = ceilings(sizes_mib=(1024,), reps=5)
+        out["p2_poisson_c2"] = p2_poisson(fh, np, dev)
+
+    # ---------------- CPU baseline: the oracle per phase on the same workload
     if not args.no_cpu:
         import oracle as O
         os.environ.pop("FH_TIME_SPMV", None)
-        K = fh.assemble_elasticity_stiffness_3d(mesh, fh.ElasticMaterial(E, nu))
-        b = fh.assemble_elasticity_body_force_3d(mesh, fbody)
-        Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
-        del K
-        cp, ri, va = Kb.export_csc()
-        A = O.csc_from_parts(cp, ri, va)
-        del cp, ri, va
-        tc = O.time_cg_iterations(A, bb, args.cpu_iters)
+        nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+            (os.cpu_count() or 1)
+        om = O.box_beam(10.0, 2.0, 2.0, nx, ny, nz)
+        ta = time.perf_counter()
+        Ko = O.assemble_elasticity(om, E, nu, nthreads=nthreads)
+        bo = O.assemble_body_force(om, fbody)
+        tb = time.perf_counter()
+        fx = O.clamp_dofs_x0(om)
+        Kbo, bbo = O.apply_dirichlet_bc(Ko, bo, fx, np.zeros(len(fx)))
+        tcd = time.perf_counter()
+        del Ko
+        tc = O.time_cg_iterations(Kbo, bbo, args.cpu_iters)
+        per_it = tc / args.cpu_iters
         out["cpu_baseline"] = {
             "value": n * args.cpu_iters / tc, "unit": "DOF·iter/s", "cores": 1,
             "kind": "port",
             "sample": f"{args.cpu_iters} Jacobi-PCG iterations of the cg.rs loop (CSC "
-                      f"column-scatter SpMV, sequential dots) on the same condensed {n}-DOF "
-                      f"matrix, one host thread ({os.cpu_count()} logical CPUs present)",
+                      f"column-scatter SpMV, sequential dots, one host thread) on the "
+                      f"condensed {n}-DOF cantilever; the full assembly ({nthreads} threads) "
+                      f"and condensation (sequential) of the same system timed beside it",
             "seconds": tc,
+            "phases": {
+                "assembly_s": tb - ta, "assembly_threads": nthreads,
+                "condensation_s": tcd - tb, "cg_s_per_iteration": per_it,
+                "host_logical_cpus": os.cpu_count(),
+                "solve_estimate_s": (tb - ta) + (tcd - tb) + per_it * int(sref.iterations),
+                "solve_estimate_note": "assembly + condensation + the reference stopping "
+                                       "rule's iteration count on the GPU x CPU time per "
+                                       "iteration"},
         }
-        log(f"cpu baseline: {args.cpu_iters} it in {tc:.2f} s")
+        log(f"cpu baseline: assembly {tb - ta:.2f} s ({nthreads} threads), condensation "
+            f"{tcd - tb:.2f} s, {args.cpu_iters} CG it in {tc:.2f} s")
 
     print(json.dumps(out))
 

@@ -5,8 +5,9 @@
 // E = 1) and J = g^T u with K u = f, the adjoint K lambda = g gives
 // dJ/dE_c = -lambda_c^T K0_c u_c; for the compliance J = f^T u, lambda = u.
 // k_cell_bilinear_k0 evaluates -w_c^T K0_c u_c for every cell (one thread per
-// cell; reads 4 vertices and 2 x 12 nodal values, writes one double), in the
-// oracle's loop order, so the FP64 result is bit-identical to it.
+// cell; reads 4 vertices and 2 x 12 nodal values, writes one double) in
+// strain form, in the oracle's operation order, so the FP64 result is
+// bit-identical to it.
 #include "fh_geom.hpp"
 #include "fh_internal.hpp"
 
@@ -14,6 +15,25 @@ namespace fh {
 
 using namespace fhg;
 
+// Displacement gradient G[a][b] = sum_{i=1..3} (v_i[a] - v_0[a]) g_i[b] of the
+// nodal values v of one cell: sum_i g_i = 0, so subtracting vertex 0's value
+// leaves the same gradient and removes the rigid translation before any
+// rounding (the energy of a cell that mostly translates would otherwise be
+// the small difference of large products).
+__device__ __forceinline__ void cell_grad_u(const double v[4][3], const double g[4][3],
+                                            double G[3][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            G[a][b] = ((v[1][a] - v[0][a]) * g[1][b] + (v[2][a] - v[0][a]) * g[2][b]) +
+                      (v[3][a] - v[0][a]) * g[3][b];
+}
+
+// -w_c^T K0_c u_c in strain form: w^T K0 u = vol (lambda tr(e_w) tr(e_u) +
+// 2 mu e_w : e_u), e = (G + G^T) / 2, which is the elasticity.rs:133-136
+// element matrix contracted with w and u (algebraically identical), without
+// its cancellation.  Same operation order as the oracle.
 __global__ __launch_bounds__(256) void k_cell_bilinear_k0(const double* __restrict__ V,
                                                           const uint32_t* __restrict__ C,
                                                           uint32_t nc, double nu,
@@ -42,17 +62,18 @@ __global__ __launch_bounds__(256) void k_cell_bilinear_k0(const double* __restri
             wl[i][a] = w[3 * (uint64_t)t[i] + a];
             ul[i][a] = u[3 * (uint64_t)t[i] + a];
         }
-    double acc = 0.0;
+    double Gw[3][3], Gu[3][3];
+    cell_grad_u(wl, g, Gw);
+    cell_grad_u(ul, g, Gu);
+    double ee = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-                    acc += wl[i][a] * (elas_entry(g[i], g[j], a, b, lambda, mu, vol) * ul[j][b]);
-    out[c] = -acc;
+        for (int b = 0; b < 3; ++b)
+            ee += (0.5 * (Gw[a][b] + Gw[b][a])) * (0.5 * (Gu[a][b] + Gu[b][a]));
+    const double trw = (Gw[0][0] + Gw[1][1]) + Gw[2][2];
+    const double tru = (Gu[0][0] + Gu[1][1]) + Gu[2][2];
+    out[c] = -(vol * (lambda * trw * tru + 2.0 * mu * ee));
 }
 
 // elasticity.rs:240-318: per cell stress[9], strain[9], von Mises, trace.

@@ -609,6 +609,7 @@ fh_matrix* assemble_elasticity(fh_mesh* m, double E, double nu, const double* Ec
     auto pat = mesh_pattern(m);
     std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 3));
     assemble_elasticity_into(m, E, nu, Ecell_dev, a.get(), d.stream);
+    a->skipped_cells = mesh_degenerate_cells(m);
     return a.release();
 }
 
@@ -625,6 +626,7 @@ fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density) {
     FH_HIP(hipStreamSynchronize(d.stream));
     a->comp_diag = true;
     a->nnz_scalar = a->pat->nnzb * 3;  // one entry per component of each block
+    a->skipped_cells = mesh_degenerate_cells(m);
     return a.release();
 }
 
@@ -723,6 +725,7 @@ fh_matrix* apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix
     o->nnz_scalar = 0;
     o->condensed = true;
     o->comp_diag = a->comp_diag;
+    o->skipped_cells = a->skipped_cells;
     o->vals.alloc(dev, a->vals.n);
     d2d(o->vals.p, a->vals.p, a->vals.n, s);
     o->fixed.alloc(dev, a->n);
@@ -801,6 +804,54 @@ HostCsr export_scalar_csr(const fh_matrix* a) {
     return h;
 }
 
+// cells whose affine map the assembly rejects (jacobian.rs:29-31), counted
+// with the same geometry routines the assembly kernels use
+template <int D>
+__global__ __launch_bounds__(256) void k_count_degenerate(const double* __restrict__ V,
+                                                          const uint32_t* __restrict__ C,
+                                                          uint32_t nc, int K,
+                                                          uint32_t* __restrict__ count) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    bool bad = false;
+    if (c < nc) {
+        double x[4][3];
+#pragma unroll
+        for (int a = 0; a < D + 1; ++a)
+#pragma unroll
+            for (int q = 0; q < D; ++q) x[a][q] = V[(uint64_t)C[(uint64_t)c * K + a] * D + q];
+        if constexpr (D == 2) {
+            Map2 m;
+            bad = !affine2(x[0], x[1], x[2], m);
+        } else {
+            Map3 m;
+            bad = !affine3(x[0], x[1], x[2], x[3], m);
+        }
+    }
+    const uint64_t mask = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, (uint32_t)__popcll(mask));
+}
+
+uint64_t mesh_degenerate_cells(fh_mesh* m) {
+    if (m->n_degenerate >= 0) return (uint64_t)m->n_degenerate;
+    hipStream_t s = device(m->dev).stream;
+    DBuf<uint32_t> cnt(m->dev, 1);
+    fill_u32(cnt.p, 1, 0u, s);
+    if (m->nc) {
+        if (m->dim == 2)
+            hipLaunchKernelGGL(k_count_degenerate<2>, dim3(cdiv(m->nc, 256)), dim3(256), 0, s,
+                               m->verts.p, m->cells.p, m->nc, m->k, cnt.p);
+        else
+            hipLaunchKernelGGL(k_count_degenerate<3>, dim3(cdiv(m->nc, 256)), dim3(256), 0, s,
+                               m->verts.p, m->cells.p, m->nc, m->k, cnt.p);
+        FH_LAUNCH_CHECK("count_degenerate");
+    }
+    uint32_t h = 0;
+    d2h(&h, cnt.p, 1, s);
+    FH_HIP(hipStreamSynchronize(s));
+    m->n_degenerate = h;
+    return h;
+}
+
 // kind 0: stiffness (poisson.rs:23-69 / 124-165), kind 1: mass (mass.rs:10-82)
 fh_matrix* assemble_p1(fh_mesh* m, int kind) {
     hipStream_t s = device(m->dev).stream;
@@ -828,6 +879,7 @@ fh_matrix* assemble_p1(fh_mesh* m, int kind) {
                                a->vals.p);
     }
     FH_LAUNCH_CHECK("asm_p1");
+    a->skipped_cells = mesh_degenerate_cells(m);
     return a.release();
 }
 
@@ -996,6 +1048,7 @@ int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out) {
         }
         if (method != 0) assemble_p2_mfma(m, a.get(), method == 1, s);
         FH_HIP(hipStreamSynchronize(s));
+        a->skipped_cells = mesh_degenerate_cells(m);
         *out = a.release();
     });
 }
@@ -1061,6 +1114,13 @@ int fh_apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
         *out = apply_dirichlet(a, bi.p, nfix, dofs, vals, bo.p, s);
         d2h(b_out, bo.p, a->n, s);
         FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_matrix_skipped_cells(const fh_matrix* a, uint64_t* count) {
+    return guard([&] {
+        FH_CHECK(a && count, "null argument");
+        *count = a->skipped_cells;
     });
 }
 

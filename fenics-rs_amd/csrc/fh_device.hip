@@ -201,6 +201,54 @@ uint64_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, int dev
     return t;
 }
 
+// ---------------------------------------------------------------- ceilings
+// Bandwidth ceilings of the device: a read stream (16-byte non-temporal
+// loads, the form of the CG matrix stream; one partial per block is written so
+// the loads stay live), a copy, and a write stream.  Grid-stride loops over
+// 16-byte elements, four independent loads in flight per thread.
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void k_stream_read(const dbl2_t* __restrict__ a, uint64_t n,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[4];
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+    dbl2_t s0 = {0.0, 0.0}, s1 = s0, s2 = s0, s3 = s0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        s0 += __builtin_nontemporal_load(a + i);
+        s1 += __builtin_nontemporal_load(a + i + stride);
+        s2 += __builtin_nontemporal_load(a + i + 2 * stride);
+        s3 += __builtin_nontemporal_load(a + i + 3 * stride);
+    }
+    for (; i < n; i += stride) s0 += __builtin_nontemporal_load(a + i);
+    const dbl2_t t = (s0 + s1) + (s2 + s3);
+    const double v = fhd::block_sum256(t.x + t.y, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const dbl2_t* __restrict__ a,
+                                                     dbl2_t* __restrict__ b, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const dbl2_t x0 = __builtin_nontemporal_load(a + i);
+        const dbl2_t x1 = __builtin_nontemporal_load(a + i + stride);
+        const dbl2_t x2 = __builtin_nontemporal_load(a + i + 2 * stride);
+        const dbl2_t x3 = __builtin_nontemporal_load(a + i + 3 * stride);
+        b[i] = x0;
+        b[i + stride] = x1;
+        b[i + 2 * stride] = x2;
+        b[i + 3 * stride] = x3;
+    }
+    for (; i < n; i += stride) b[i] = __builtin_nontemporal_load(a + i);
+}
+
+__global__ __launch_bounds__(256) void k_stream_write(dbl2_t* __restrict__ b, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const dbl2_t v = {1.0, 2.0};
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) b[i] = v;
+}
+
 }  // namespace fh
 
 using namespace fh;
@@ -241,6 +289,44 @@ int fh_best_device(int* index) {
     return guard([&] {
         FH_CHECK(index, "null output");
         *index = resolve_device(-1);
+    });
+}
+
+int fh_stream_bandwidth(int dev, uint64_t bytes, int kind, int reps, double* gbps,
+                        double* best_ms) {
+    return guard([&] {
+        FH_CHECK(gbps && best_ms, "null output");
+        FH_CHECK(kind >= 0 && kind <= 2, "kind must be 0 (read), 1 (copy) or 2 (write)");
+        FH_CHECK(bytes >= 4096 && reps >= 1, "bytes must be at least 4096, reps at least 1");
+        DeviceCtx& d = device(dev);
+        hipStream_t s = d.stream;
+        const uint64_t n = bytes / 16;
+        DBuf<dbl2_t> a(dev, n), b;
+        if (kind == 1) b.alloc(dev, n);
+        const unsigned grid = (unsigned)std::min<uint64_t>(cdiv(n, 256), 8u * 256u * (unsigned)d.n_cu / 4u);
+        DBuf<double> part(dev, grid);
+        fill_f64(reinterpret_cast<double*>(a.p), 2 * n, 1.0, s);
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int r = 0; r <= reps; ++r) {  // the first launch is a warm-up
+            FH_HIP(hipEventRecord(e0, s));
+            if (kind == 0) hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, s, a.p, n, part.p);
+            else if (kind == 1) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, a.p, b.p, n);
+            else hipLaunchKernelGGL(k_stream_write, dim3(grid), dim3(256), 0, s, a.p, n);
+            FH_LAUNCH_CHECK("stream");
+            FH_HIP(hipEventRecord(e1, s));
+            FH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0) best = std::min(best, ms);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double moved = (double)n * 16.0 * (kind == 1 ? 2.0 : 1.0);
+        *best_ms = best;
+        *gbps = moved / (best * 1e-3) / 1e9;
     });
 }
 

@@ -75,6 +75,7 @@ struct fh_mesh {
     std::unique_ptr<fh::Adjacency> adj;
     std::shared_ptr<fh::Pattern> pat;   // vertex graph pattern
     std::unique_ptr<fh::MeshP2> p2;
+    int64_t n_degenerate = -1;  // cells with |det J| < 1e-14 (counted once, lazily)
 };
 
 struct fh_matrix {
@@ -96,6 +97,7 @@ struct fh_matrix {
     bool comp_diag = false;  // blocks couple equal components only (elasticity mass):
                              // the export leaves out the off-diagonal components
     fh::DBuf<uint32_t> fixed_count;  // duplicates in the prescribed list (diag value)
+    uint64_t skipped_cells = 0;      // cells the assembly skipped (|det J| < 1e-14)
 };
 
 namespace fh {
@@ -140,6 +142,8 @@ fh_matrix* matrix_from_csr(int dev, uint32_t nrows, int bs, const uint32_t* row_
 fh_matrix* matrix_from_csc(int dev, uint64_t n, const uint64_t* col_ptr, const uint64_t* row_idx,
                            const double* vals);
 void ensure_vals32(fh_matrix* a);
+// cells of the mesh that the assembly skips (jacobian.rs:29-31: |det J| < 1e-14)
+uint64_t mesh_degenerate_cells(fh_mesh* m);
 // scalar CSR of the operator the matrix exports (condensed couplings, folded
 // zeros and the off-diagonal components of a component-diagonal mass left out)
 struct HostCsr {

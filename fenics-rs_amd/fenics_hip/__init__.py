@@ -109,6 +109,8 @@ def lib():
         "fh_get_device_info": (C.c_int, [C.c_int, C.POINTER(DeviceInfo)]),
         "fh_best_device": (C.c_int, [C.POINTER(C.c_int)]),
         "fh_synchronize": (C.c_int, [C.c_int]),
+        "fh_stream_bandwidth": (C.c_int, [C.c_int, u64, C.c_int, C.c_int, C.POINTER(dbl),
+                                          C.POINTER(dbl)]),
         "fh_mesh_create": (C.c_int, [C.c_int, u64, vp, u64, vp, C.c_int, PP]),
         "fh_mesh_unit_square": (C.c_int, [u64, u64, C.c_int, PP]),
         "fh_mesh_box_beam": (C.c_int, [dbl, dbl, dbl, u64, u64, u64, C.c_int, PP]),
@@ -132,6 +134,7 @@ def lib():
         "fh_apply_dirichlet": (C.c_int, [vp, vp, u64, vp, vp, PP, vp]),
         "fh_apply_dirichlet_dev": (C.c_int, [vp, vp, u64, vp, vp, PP, vp]),
         "fh_matrix_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_int)]),
+        "fh_matrix_skipped_cells": (C.c_int, [vp, C.POINTER(u64)]),
         "fh_matrix_export_csc": (C.c_int, [vp, vp, vp, vp]),
         "fh_matrix_export_bsr3": (C.c_int, [vp, vp, vp, vp, vp]),
         "fh_matrix_spmv": (C.c_int, [vp, vp, vp, C.c_int]),
@@ -410,6 +413,12 @@ class SparseMatrix:
 
     def nnz(self):
         return self._info()[1]
+
+    def skipped_cells(self) -> int:
+        """Cells the assembly skipped (|det J| < 1e-14, jacobian.rs:29-31)."""
+        c = C.c_uint64()
+        _check(lib().fh_matrix_skipped_cells(self.h, C.byref(c)))
+        return int(c.value)
 
     def export_csc(self):
         """(col_ptr u64[n+1], row_idx u64[nnz], values f64[nnz]) — faer's canonical CSC."""
@@ -1162,3 +1171,12 @@ def clamp_dofs_x0(vertices, tol=1e-6):
     """elasticity_tests.rs:94-101: all three dofs of the vertices with x < 1e-6."""
     idx = np.nonzero(np.asarray(vertices)[:, 0] < tol)[0]
     return VectorDirichletBC([(3 * int(i) + a, 0.0) for i in idx for a in range(3)])
+
+
+def stream_bandwidth(nbytes: int, kind: str = "read", reps: int = 5, device: int = 0):
+    """Measured bandwidth ceiling of the device (fh_stream_bandwidth):
+    kind 'read', 'copy' (read + write bytes) or 'write'.  Returns (GB/s, ms)."""
+    k = {"read": 0, "copy": 1, "write": 2}[kind]
+    g, ms = C.c_double(), C.c_double()
+    _check(lib().fh_stream_bandwidth(device, int(nbytes), k, reps, C.byref(g), C.byref(ms)))
+    return g.value, ms.value

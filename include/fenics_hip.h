@@ -100,6 +100,12 @@ int fh_device_count(int* count);
 int fh_get_device_info(int index, fh_device_info* out);      /* gpu.rs:178-234 discover_gpus */
 int fh_best_device(int* index);                           /* gpu.rs:237-242 best_gpu_index */
 int fh_synchronize(int device);
+/* Measurement utility (new; the reference prints per-solve GB/s only,
+ * gpu.rs:1119-1129): the best of `reps` event-timed passes of a read stream
+ * (kind 0, 16-byte non-temporal loads), a copy (kind 1; read + write bytes
+ * counted) or a write stream (kind 2) over `bytes` of device memory.       */
+int fh_stream_bandwidth(int device, uint64_t bytes, int kind, int reps, double* gbps,
+                        double* best_ms);
 
 /* ------------------------------------------------------------------ mesh
  * replaces SimplicialMesh<DIM,NODES> (crates/fenics-mesh/src/mesh.rs:10-20);
@@ -161,6 +167,9 @@ int fh_apply_dirichlet(const fh_matrix* a, const double* b_in, uint64_t nfix,
 int fh_apply_dirichlet_dev(const fh_matrix* a, const double* b_in_dev, uint64_t nfix,
                            const uint64_t* dofs, const double* vals, fh_matrix** out,
                            double* b_out_dev);
+/* cells the assembly of `a` skipped because |det J| < 1e-14 (the silent
+ * skip of jacobian.rs:29-31 / poisson.rs:42-45, counted; new)              */
+int fh_matrix_skipped_cells(const fh_matrix* a, uint64_t* count);
 /* matrix shape, scalar nnz of the canonical CSC, block size */
 int fh_matrix_info(const fh_matrix* a, uint64_t* n, uint64_t* nnz, int* block);
 /* faer SparseColMat raw parts: col_ptr[n+1], row_idx[nnz], values[nnz]     */

@@ -503,10 +503,19 @@ double or_l2_error_3d(const double* V, uint64_t nc, const uint64_t* C, const dou
     return std::sqrt(tot);
 }
 
-// Adjoint of the compliance (new; pinned by finite differences and the
-// homogeneity identity sum_c E_c dJ/dE_c = -J):
-// out[c] = -sum_{i,j,a,b} w[3t_i+a] * (K0(ia,jb) * u[3t_j+b]) with K0 the
-// elasticity.rs:133-136 element matrix at E = 1.
+// Adjoint of the compliance (new; pinned by finite differences, the
+// homogeneity identity sum_c E_c dJ/dE_c = -J and exact rational arithmetic):
+// out[c] = -w_c^T K0_c u_c with K0 the elasticity.rs:133-136 element matrix at
+// E = 1, evaluated in strain form, vol (lambda tr(e_w) tr(e_u) + 2 mu e_w:e_u)
+// with e = sym(G), G[a][b] = sum_{i=1..3} (v_i[a] - v_0[a]) g_i[b] (the same
+// contraction; the differences remove the rigid translation before rounding).
+static void or_cell_grad(const double v[4][3], const double g[4][3], double G[3][3]) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            G[a][b] = ((v[1][a] - v[0][a]) * g[1][b] + (v[2][a] - v[0][a]) * g[2][b]) +
+                      (v[3][a] - v[0][a]) * g[3][b];
+}
+
 int or_cell_bilinear_k0(const double* V, uint64_t nc, const uint64_t* C, double nu,
                         const double* w, const double* u, double* out) {
     const double lambda = lame_lambda(1.0, nu), mu = lame_mu(1.0, nu);
@@ -520,14 +529,22 @@ int or_cell_bilinear_k0(const double* V, uint64_t nc, const uint64_t* C, double 
         const double vol = (1.0 / 6.0) * std::fabs(m.det);
         double g[4][3];
         for (int i = 0; i < 4; ++i) grad3(m, P1_TET_GRAD[i], g[i]);
-        double acc = 0.0;
+        double wl[4][3], ul[4][3];
         for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j)
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        acc += w[3 * t[i] + a] *
-                               (elas_entry(g[i], g[j], a, b, lambda, mu, vol) * u[3 * t[j] + b]);
-        out[c] = -acc;
+            for (int a = 0; a < 3; ++a) {
+                wl[i][a] = w[3 * t[i] + a];
+                ul[i][a] = u[3 * t[i] + a];
+            }
+        double Gw[3][3], Gu[3][3];
+        or_cell_grad(wl, g, Gw);
+        or_cell_grad(ul, g, Gu);
+        double ee = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                ee += (0.5 * (Gw[a][b] + Gw[b][a])) * (0.5 * (Gu[a][b] + Gu[b][a]));
+        const double trw = (Gw[0][0] + Gw[1][1]) + Gw[2][2];
+        const double tru = (Gu[0][0] + Gu[1][1]) + Gu[2][2];
+        out[c] = -(vol * (lambda * trw * tru + 2.0 * mu * ee));
     }
     return 0;
 }

@@ -237,3 +237,43 @@ def test_unstructured_mesh_bitwise(dim):
         Kco, bco = O.apply_dirichlet_bc(Ko, bo, [d for d, _ in fixed], [v for _, v in fixed])
         same_csc(Kc, Kco)
         assert np.array_equal(bits(bc), bits(bco))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_degenerate_cells_are_skipped_and_counted(dim):
+    """jacobian.rs:29-31 / poisson.rs:42-45: a cell with |det J| < 1e-14 adds
+    nothing.  Two cells with a repeated vertex (det J = 0 exactly) appended to
+    a mesh leave every matrix and load bitwise the oracle's, and the assembly
+    reports them as skipped.  The repeated vertices couple only vertices that
+    other cells couple already, so the pattern is the reference's."""
+    base = O.unit_square(6, 5) if dim == 2 else O.box_beam(1.0, 0.5, 0.25, 5, 3, 2)
+    C = base.cells.astype(np.uint64)
+    k = C.shape[1]
+    extra = []
+    for c in (C[3], C[len(C) // 2]):
+        d = c.copy()
+        d[k - 1] = d[0]
+        extra.append(d)
+    om = O.Mesh(base.vertices.copy(), np.concatenate([C, np.array(extra, dtype=np.uint64)]))
+    gm = fh.Mesh.from_arrays(om.vertices, om.cells)
+    K = fh.assemble_stiffness(gm)
+    assert K.skipped_cells() == 2
+    same_csc(K, O.assemble_stiffness(om))
+    xo = O.quad_points(om, 2)
+    fq = np.cos(2 * xo[..., 0]) + xo[..., 1]
+    assert np.array_equal(bits(fh.assemble_rhs(gm, fq)), bits(O.assemble_rhs(om, fq)))
+    if dim == 2:
+        M = fh.assemble_mass(gm)
+        assert M.skipped_cells() == 2
+        same_csc(M, O.assemble_mass_2d(om))
+    else:
+        Ke = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.3))
+        assert Ke.skipped_cells() == 2
+        same_csc(Ke, O.assemble_elasticity(om, 7e10, 0.3))
+        b = fh.assemble_elasticity_body_force_3d(gm, [1.0, -2.0, 3.0])
+        assert np.array_equal(bits(b), bits(O.assemble_body_force(om, [1.0, -2.0, 3.0])))
+        bc = fh.clamp_dofs_x0(gm.vertices)
+        Kc, _ = fh.apply_vector_dirichlet_bc(Ke, b, bc)
+        assert Kc.skipped_cells() == 2
+    # a regular mesh skips nothing
+    assert fh.assemble_stiffness(fh.Mesh.from_arrays(base.vertices, base.cells)).skipped_cells() == 0

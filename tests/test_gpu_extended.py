@@ -242,3 +242,58 @@ def test_c3_full_size_adjoint_vs_fd():
             um, _ = prob.solve(Em)
             fd = float(prob.cell_gradient(up + um, u)[c]) / 2.0
             assert abs(fd - float(g[c])) < 1e-6 * abs(float(g[c])), (c, fd, float(g[c]))
+
+
+def test_adjoint_vs_fd_of_compliance_solves():
+    """dJ/dE_c of the adjoint kernel against central differences of the
+    compliance itself, J(E +- h e_c) = f^T u(E +- h e_c), h = 1e-5 E_c, on a
+    20x4x4 beam clamped at both ends with
+    seeded per-cell moduli.  No part of this check uses the cell
+    bilinear-form kernel: each u comes from an assembly with the perturbed
+    per-cell E and a double-double solve (residual < 1e-20), and the
+    difference J+ - J- = f^T ((h+ - h-) + (l+ - l-)) is formed from the two
+    double-double solutions.  What limits it is the assembled matrices
+    themselves: their FP64 rounding (~1e-16 |K|) does not annihilate rigid
+    motions, so J carries ~1e-16 |K| |u|^2 of noise; with both ends clamped
+    the rigid part of u stays of the size of its deformation and that noise
+    stays ~1e-7 of the differences even for the cells of least strain energy
+    at h = 1e-5 E_c (a cantilever's free end moves as a rigid body, and there
+    it reached 1e-4 at h = 1e-6 E_c).  The central-difference truncation,
+    (h/E)^2 times the cell's J'''/J' E^2 (~1e2 measured), is ~1e-8."""
+    import math
+    import torch
+    L_ = 10.0
+    m = fh.box_beam(L_, 2.0, 2.0, 20, 4, 4)
+    rng = np.random.default_rng(44)
+    E = 210e9 * np.exp(0.3 * rng.standard_normal(m.nc))
+    nu, f3 = 0.3, [0.0, 0.0, -7.85e4]
+    V = m.vertices
+    ends = np.nonzero((V[:, 0] < 1e-6) | (V[:, 0] > L_ - 1e-6))[0]
+    bc = fh.VectorDirichletBC([(3 * int(i) + a, 0.0) for i in ends for a in range(3)])
+    b = fh.assemble_elasticity_body_force_3d(m, f3)
+
+    def solve(Ec):
+        K = fh.assemble_elasticity_stiffness_3d(m, fh.ElasticMaterial(210e9, nu), E_cell=Ec)
+        Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
+        bt = torch.tensor(bb, device="cuda:0")
+        xh, xl = torch.empty_like(bt), torch.empty_like(bt)
+        rs = Kb.solve_refined_dev(bt, xh, xl, target=1e-20, max_outer=10, inner_tol=1e-6,
+                                  max_iter=20000)
+        assert rs.rel_residual_pair < 1e-20
+        return bb, xh.cpu().numpy(), xl.cpu().numpy()
+
+    bb, h0, l0 = solve(E)
+    u = h0 + l0
+    J0 = math.fsum(np.concatenate([bb * h0, bb * l0]))
+    g = fh.cell_bilinear_k0(m, nu, u, u)  # the adjoint kernel: -u_c^T K0_c u_c = dJ/dE_c
+    assert abs(float(np.dot(E, g)) + J0) < 1e-10 * abs(J0)
+    for c in rng.choice(m.nc, 16, replace=False):
+        Ep, Em = E.copy(), E.copy()
+        Ep[c] += 1e-5 * E[c]
+        Em[c] -= 1e-5 * E[c]
+        h = 0.5 * (Ep[c] - Em[c])  # the step actually taken in FP64
+        _, hp, lp = solve(Ep)
+        _, hm, lm = solve(Em)
+        dJ = math.fsum(np.concatenate([bb * (hp - hm), bb * (lp - lm)]))
+        fd = dJ / (2.0 * h)
+        assert abs(fd - g[c]) < 1e-6 * abs(g[c]), (int(c), fd, float(g[c]))

@@ -599,3 +599,57 @@ def test_exact_residual_checker_vs_fractions():
         rf = np.array([float(v) for v in r]).astype(np.longdouble)
         ref = float(np.sqrt((rf ** 2).sum()) / np.sqrt((bb.astype(np.longdouble) ** 2).sum()))
         assert exact_residual(Kb, bb, x) == ref
+
+
+def test_adjoint_contraction_vs_exact_rationals():
+    """-w_c^T K0_c u_c (the adjoint kernel's quantity) against exact rational
+    arithmetic: the affine map, gradients, Lame constants and the
+    elasticity.rs:133-136 element matrix contracted with w and u, all in
+    Fractions from the FP64 inputs.  Fields dominated by a rigid motion (the
+    situation near a cantilever's free end) leave a strain energy many orders
+    below |u|^2 |K0|; the oracle's strain-form evaluation loses only about
+    eps * |grad u| / |strain| of it (the rotation left in the gradient), where
+    contracting the FP64 element matrix itself loses about
+    eps * |u|^2 |K0| / (u^T K0 u), the square of that ratio times more."""
+    from fractions import Fraction as Fr
+    m = O.box_beam(1.0, 0.4, 0.3, 2, 2, 1)
+    nu = 0.3
+    rng = np.random.default_rng(8)
+    R = np.array([[0.0, -1.0, 0.5], [1.0, 0.0, -0.2], [-0.5, 0.2, 0.0]])  # skew: a rotation
+    X = m.vertices
+    for scale in (1.0, 1e-6, 1e-9):
+        u = (np.array([3.0, -2.0, 1.0]) + X @ R.T) + scale * rng.standard_normal(X.shape)
+        u = u.reshape(-1)
+        w = u.copy()
+        got = O.cell_bilinear_k0(m, nu, w, u)
+        nuq = Fr(nu)
+        lam = nuq / ((1 + nuq) * (1 - 2 * nuq))
+        mu = Fr(1) / (2 * (1 + nuq))
+        for c in range(m.nc):
+            t = [int(v) for v in m.cells[c]]
+            P = [[Fr(float(X[v][k])) for k in range(3)] for v in t]
+            J = [[P[j + 1][i] - P[0][i] for j in range(3)] for i in range(3)]
+            det = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2])
+                   - J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2])
+                   + J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]))
+            cof = [[(J[(i + 1) % 3][(j + 1) % 3] * J[(i + 2) % 3][(j + 2) % 3]
+                     - J[(i + 1) % 3][(j + 2) % 3] * J[(i + 2) % 3][(j + 1) % 3])
+                    for j in range(3)] for i in range(3)]
+            invT = [[cof[i][j] / det for j in range(3)] for i in range(3)]  # J^{-T}
+            G = [[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+            g = [[sum(invT[i][k] * G[a][k] for k in range(3)) for i in range(3)] for a in range(4)]
+            vol = abs(det) / 6
+            acc = Fr(0)
+            for i in range(4):
+                for j in range(4):
+                    gg = sum(g[i][k] * g[j][k] for k in range(3))
+                    for a in range(3):
+                        for b in range(3):
+                            K = vol * (lam * g[j][b] * g[i][a] + mu * gg * (1 if a == b else 0)
+                                       + mu * g[j][a] * g[i][b])
+                            acc += Fr(float(w[3 * t[i] + a])) * K * Fr(float(u[3 * t[j] + b]))
+            ref = -float(acc)
+            # the strain form's own bound: rounding of G (entries of the size
+            # of the rotation) relative to its symmetric part (the strain)
+            tol = 8 * 2.2e-16 * (1.0 + 1.0 / scale)
+            assert abs(got[c] - ref) <= tol * abs(ref), (scale, c, got[c], ref)
