@@ -23,6 +23,8 @@ Beside the timed steps (none of this is in `value`):
   * GPU time per phase (assembly, condensation, solve);
   * device bandwidth ceilings (read stream, copy);
   * P2 Poisson unit_cube(64^3) (2,146,689 DOF): SpMV and solve;
+  * the batched workload (256 Poisson instances x 4 right-hand sides) by
+    block CG, FP64 and mixed;
   * the CPU oracle per phase: assembly over all host threads, the sequential
     condensation and a bounded number of single-thread cg.rs iterations.
 
@@ -131,6 +133,43 @@ def p2_poisson(fh, np, dev):
                         "launches": int(st.spmv_timed)}
     log(f"C2: fp64 {out['fp64']['ms']:.1f} ms ({out['fp64']['iterations']} it), mixed "
         f"{out['mixed']['ms']:.1f} ms; SpMV {st.spmv_avg_ms * 1e3:.1f} us = {ach:.0f} GB/s")
+    return out
+
+
+def batched_c4(fh, np):
+    """C4: 256 unit_square(N_i, N_i) Poisson instances, N_i ~ U{32..512}
+    (seed 2024), four nodal loads each (b = M f), tol 1e-10: block CG over the
+    four columns, FP64 and mixed (FP32 matrix values and gathered Krylov
+    vectors), with the independent-column iteration counts of the single-system
+    CG on the largest instance beside it."""
+    rng = np.random.default_rng(2024)
+    sizes = rng.integers(32, 513, 256)
+    fs = [rng.uniform(-1, 1, ((int(N) + 1) ** 2, 4)) for N in sizes]
+    out = {"workload": "256 unit_square(N,N) Poisson P1, N~U{32..512} seed 2024, 4 RHS each, "
+                       "block CG to 1e-10", "unknowns": int(sum((int(N) + 1) ** 2 for N in sizes))}
+    for prec, name in ((fh.FP64, "fp64"), (fh.MIXED, "mixed")):
+        _, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, max_iter=40000,
+                                                 precision=prec)
+        out[name] = {"kernel_ms": st.kernel_ms, "wall_ms": st.wall_ms,
+                     "dof_iter_per_s": st.gflops * 1e9,
+                     "block_iterations_max": int(its.max()),
+                     "block_iterations_sum": int(st.replacements),
+                     "true_rel_residual_max": st.true_rel_residual,
+                     "effective_gbps": st.effective_bw_gbps,
+                     "roofline_frac": st.effective_bw_gbps / HBM_PEAK_GBS,
+                     "iter_bytes_full_batch": st.iter_bytes}
+        log(f"C4 {name}: {st.kernel_ms:.1f} ms, block iterations <= {its.max()}, "
+            f"{st.gflops:.2f} GDOF.iter/s, {st.effective_bw_gbps:.0f} GB/s effective")
+    # the largest instance, column 0, by the independent (single-system) CG
+    i = int(np.argmax(sizes))
+    N = int(sizes[i])
+    m = fh.unit_square(N, N)
+    K, M = fh.assemble_stiffness(m), fh.assemble_mass(m)
+    bd = m.boundary_dofs()
+    Kb, bb = fh.apply_dirichlet_bc(K, M.spmv(fs[i][:, 0]), fh.DirichletBC(bd, np.zeros(len(bd))))
+    _, s1 = Kb.solve(bb, tol=1e-10, max_iter=40000, precision=fh.FP64)
+    out["largest_instance"] = {"N": N, "independent_cg_iterations_col0": int(s1.iterations),
+                               "block_cg_iterations": int(its[i].max())}
     return out
 
 
@@ -354,6 +393,7 @@ def main():
 
         out["ceilings"] = ceilings(sizes_mib=(1024,), reps=5)
         out["p2_poisson_c2"] = p2_poisson(fh, np, dev)
+        out["batched_c4"] = batched_c4(fh, np)
 
     # ---------------- CPU baseline: the oracle per phase on the same workload
     if not args.no_cpu:

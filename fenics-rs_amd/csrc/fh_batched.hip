@@ -1,56 +1,68 @@
-// fh_batched.hip — many independent Poisson P1 problems, several right-hand
-// sides each, solved together (new; no counterpart in the reference, which
-// solves one system per call through solve_poisson_2d, poisson_solver.rs:26).
+// fh_batched.hip — many independent Poisson P1 problems with up to four
+// right-hand sides each, solved together by block CG (new; no counterpart in
+// the reference, which solves one system per solve_poisson_2d call,
+// poisson_solver.rs:26).
 //
-// Layout: the instances form one block-diagonal system over the disjoint union
-// of their unit_square meshes (generators.rs:24-55), assembled and condensed by
-// the ordinary single-system kernels.  Each instance's rows are padded to a
-// multiple of 256 (four 64-row slices = one workgroup) with isolated, fixed
-// rows, so every workgroup of the batched kernels belongs to exactly one
-// instance.  Vectors hold the NR right-hand sides interleaved per row
-// ([row][NR]): one SpMM pass reads each matrix entry once for all of them.
+// Layout.  The instances form one block-diagonal system over the disjoint
+// union of their unit_square meshes (generators.rs:24-55), assembled and
+// condensed by the single-system kernels.  Each instance's rows are padded to
+// a multiple of 256 (one row tile, four SELL slices) with isolated, fixed
+// rows, so every tile belongs to one instance.  Vectors hold the four
+// right-hand-side columns interleaved per row ([row][4], 32 bytes): one
+// product pass reads each matrix entry once for all of them.
 //
-// Multi-RHS CG: every (instance, rhs) column runs its own Jacobi-PCG
-// recurrences (alpha, beta per column — block CG with block size 1 per column,
-// sharing every SpMM); a converged column is frozen, a finished instance is
-// skipped.  One iteration = 4 launches over all instances (SpMM, per-instance
-// reduction, update, per-instance reduction), all arguments fixed, so a chunk
-// of iterations is captured once into a HIP graph and replayed.  Reductions
-// are fixed-order (deterministic).  Mixed precision: FP32 matrix and vectors in
-// the loop, FP64 residual replacement per instance (b - A64 (x64 + x32)),
-// convergence declared on the true FP64 residual.
+// Block CG per instance with A-orthonormal directions (O'Leary 1980 in the
+// breakdown-free form of Dubrulle 2001 / Ji & Li 2017), Jacobi D^{-1}:
+//     W = Z + P beta,  Q_W = A W,  W^T A W = L L^T,  P = W L^{-T},  Q = Q_W L^{-T},
+//     alpha = P^T R,  X += P alpha,  R -= Q alpha,  Z = D^{-1} R,  beta = -Q^T Z.
+// The columns share one Krylov space, so each converges in fewer iterations
+// than an independent CG on it; with one column this is cg.rs:40-149.  The
+// Cholesky factor drops a direction whose A-norm is not above 1e-12 of its
+// own outside the span of the others (rank-revealing deflation: the block
+// shrinks instead of breaking down), and a column whose residual reaches tol
+// leaves the block (its X and R stay; its W column is zero from then on).
+//
+// One block iteration is five launches: W = Z + P beta over the active tiles
+// (so the product gathers one vector, not two), the product Q_W = A W, a
+// one-workgroup-per-instance reduction (L, alpha), the update over the
+// active tiles, and a second reduction (convergence per column, beta).  The
+// s x s algebra runs once per instance, in fixed order, so runs are
+// bitwise reproducible.  The tile kernels have a fixed grid that strides over
+// a device-side work list of the tiles of unfinished instances, rebuilt by a
+// compaction kernel after every chunk of iterations: a chunk is captured once
+// into a HIP graph and replayed with fixed arguments, and the host queues
+// the next chunk before it looks at the previous chunk's count of unfinished
+// instances (so it never waits with the device idle).  Kernel boundaries are
+// the device-wide synchronisation: a single persistent launch whose
+// workgroups meet at barriers inside the kernel was measured 1.2-2.2x slower
+// here, because each agent-scope barrier costs an L2 write-back and
+// invalidate (buffer_wbl2 / buffer_inv) on this 8-XCD part.
+//
+// Precision: FH_PREC_FP64 reads FP64 matrix values; the mixed modes read the
+// FP32 copy (the unit-square P1 stiffness entries are small integers and
+// halves, so the copy is exact here).  Vectors, reductions and the s x s
+// algebra are FP64; convergence is on the recursive residual of each column
+// (cg.rs:118-122), and the true FP64 residual b - A x of every column is
+// evaluated at the end and reported.
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
+#include <numeric>
+#include <string>
 #include <vector>
 
 #include "fh_internal.hpp"
 
 namespace fh {
 
-constexpr int NR = 4;
-
-struct BatchState {
-    // per column (instance * NR + k)
-    double* rz;      // r.z of the current iterate
-    double* alpha;
-    double* beta;
-    double* bnorm;
-    double* r_rep;   // true residual norm at the last replacement
-    double* res;     // last residual norm
-    uint32_t* iters; // iterations when the column finished
-    uint8_t* done;   // 0 running, 1 converged, 2 not converged, 3 breakdown
-    // per instance
-    uint32_t* it;        // completed iterations
-    uint8_t* inst_done;
-    uint8_t* need_rep;
-    const uint32_t* wg_begin;  // workgroup range of the instance
-    const uint32_t* wg_end;
-    const uint32_t* wg_inst;   // instance of each workgroup
-    double tol, delta;
-    uint32_t max_iter;
-    uint32_t mixed, restart;
-};
+constexpr int NR = 4;    // columns stored per row (block size s <= NR)
+constexpr int BT = 256;  // threads per workgroup (4 waves)
+constexpr int TS = 16;   // SELL slices per row tile: each wave takes 4 of them
+constexpr int TR = TS * 64;  // rows per tile (each lane accumulates 4 rows before the reduction)
+constexpr int NSYM = 10; // entries of a symmetric 4 x 4 matrix
 
 template <class T>
 struct SellB {
@@ -62,275 +74,556 @@ struct SellB {
     uint32_t nslices;
 };
 
-// Ap = A (z + beta p_old) for the columns of one instance; partial p.Ap.
-template <class T>
-__global__ __launch_bounds__(256) void k_b_spmm(SellB<T> A, const T* __restrict__ z, T* p0, T* p1,
-                                                T* __restrict__ Ap, BatchState S,
-                                                double* __restrict__ part) {
-    __shared__ double sh[4];
-    const uint32_t w = blockIdx.x, inst = S.wg_inst[w];
-    if (S.inst_done[inst] || S.need_rep[inst]) return;
-    const uint32_t j = S.it[inst] + 1;
-    const T* pold = (j & 1) ? p0 : p1;
-    T* pnew = (j & 1) ? p1 : p0;
-    T bt[NR];
+__device__ __forceinline__ int sym(int a, int b) {  // packed index of (a, b) in a symmetric 4 x 4
+    const int i = a < b ? a : b, j = a < b ? b : a;
+    return i * NR - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Sum over the workgroup of n values per thread (fixed butterfly + wave
+// order); every thread gets the sums.
+template <int N>
+__device__ __forceinline__ void wg_sum(double* v, double* sh /* [4][N] */) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) bt[k] = (T)S.beta[inst * NR + k];
-    const uint32_t s = w * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    double dot[NR] = {0, 0, 0, 0};
-    if (s < A.nslices) {
-        const uint32_t row = s * SLICE + lane;
-        const uint32_t base = A.soff[s], wd = (A.soff[s + 1] - base) / SLICE;
-        const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        T acc[NR] = {0, 0, 0, 0};
-        for (uint32_t q = 0; q < wd; ++q) {
-            if (q < len) {
-                const uint64_t e = (uint64_t)base + (uint64_t)q * SLICE + lane;
-                const uint32_t c = A.cols[e];
-                const T v = A.vals[e];
+    for (int k = 0; k < N; ++k) v[k] = fhd::wave_sum(v[k]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
 #pragma unroll
-                for (int k = 0; k < NR; ++k) {
-                    const T pc = fma(bt[k], pold[(uint64_t)c * NR + k], z[(uint64_t)c * NR + k]);
-                    acc[k] = fma(v, pc, acc[k]);
-                }
-            }
-        }
-        if (row < A.nrows) {
+        for (int k = 0; k < N; ++k) sh[w * N + k] = v[k];
+    __syncthreads();
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                const uint64_t i = (uint64_t)row * NR + k;
-                const T pn = fma(bt[k], pold[i], z[i]);
-                pnew[i] = pn;
-                Ap[i] = acc[k];
-                dot[k] = (double)pn * (double)acc[k];
-            }
+    for (int k = 0; k < N; ++k) v[k] = (sh[k] + sh[N + k]) + (sh[2 * N + k] + sh[3 * N + k]);
+    __syncthreads();
+}
+
+// Sum of the partial records [slot0, slot0 + w) of width N (fixed order).
+template <int N>
+__device__ __forceinline__ void reduce_slots(const double* part, uint32_t slot0, uint32_t w,
+                                             double* out, double* sh) {
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = 0.0;
+    for (uint32_t t = threadIdx.x; t < w; t += BT)
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] += part[(uint64_t)(slot0 + t) * N + k];
+    wg_sum<N>(v, sh);
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = v[k];
+}
+
+// Cholesky factor of a symmetric 4 x 4 (packed) that drops dependent
+// directions: column j is kept when the part of it not in the span of the
+// kept earlier ones, d_j = G_jj - sum_k L_jk^2, exceeds 1e-12 G_jj (and is
+// positive); a dropped direction has a zero column and row in L.  Returns the
+// kept mask.  Only directions in `cand` are considered.
+__device__ uint32_t chol_drop4(const double* g, uint32_t cand, double L[NR][NR]) {
+    uint32_t kept = 0;
+    for (int i = 0; i < NR; ++i)
+        for (int j = 0; j < NR; ++j) L[i][j] = 0.0;
+    for (int j = 0; j < NR; ++j) {
+        if (!((cand >> j) & 1)) continue;
+        const double gjj = g[sym(j, j)];
+        double d = gjj;
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        if (!(gjj > 0.0) || !(d > 1e-12 * gjj)) continue;
+        const double l = sqrt(d);
+        L[j][j] = l;
+        kept |= 1u << j;
+        for (int i = j + 1; i < NR; ++i) {
+            if (!((cand >> i) & 1)) continue;
+            double t = g[sym(i, j)];
+            for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+            L[i][j] = t / l;
         }
     }
+    return kept;
+}
+
+// v = L^{-1} u for a row vector u of the W basis (the P = W L^{-T} change of
+// basis, one row): forward substitution over the kept directions.
+__device__ __forceinline__ void lsolve_row(const double L[NR][NR], uint32_t kept, const double* u,
+                                           double* v) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const double d = fhd::block_sum256(dot[k], sh);
-        if (threadIdx.x == 0) part[(uint64_t)w * NR + k] = d;
+    for (int j = 0; j < NR; ++j) {
+        double t = u[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) t = fma(-L[j][k], v[k], t);
+        v[j] = ((kept >> j) & 1) ? t / L[j][j] : 0.0;
     }
 }
 
-// alpha per column from the instance's p.Ap partials (one block per instance).
-__global__ __launch_bounds__(256) void k_b_red1(BatchState S, const double* __restrict__ part) {
-    __shared__ double sh[4];
-    const uint32_t inst = blockIdx.x;
-    if (S.inst_done[inst] || S.need_rep[inst]) return;
-    const uint32_t w0 = S.wg_begin[inst], w1 = S.wg_end[inst];
+__device__ __forceinline__ void ld4(const double* p, double* v) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 a = *reinterpret_cast<const d2*>(p), b = *reinterpret_cast<const d2*>(p + 2);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+__device__ __forceinline__ void st4(double* p, const double* v) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    d2 a, b;
+    a.x = v[0];
+    a.y = v[1];
+    b.x = v[2];
+    b.y = v[3];
+    *reinterpret_cast<d2*>(p) = a;
+    *reinterpret_cast<d2*>(p + 2) = b;
+}
+
+// Position of this workgroup in a stride loop over a grid that is a multiple
+// of 8: the hardware deals workgroup b to XCD b mod 8, so XCD x gets the
+// contiguous positions [x G/8, (x+1) G/8) of every stride: consecutive tiles
+// (grid neighbours whose (z, p) rows the product gathers) share one L2.
+__device__ __forceinline__ uint32_t xcd_pos() {
+    const uint32_t per = gridDim.x / 8;
+    return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+// four values of a vector row as doubles, and back (rounded to TV)
+__device__ __forceinline__ void ld4(const float* p, double* v) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 a = *reinterpret_cast<const f4*>(p);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+}
+__device__ __forceinline__ void st4(float* p, const double* v) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 a;
+    a.x = (float)v[0];
+    a.y = (float)v[1];
+    a.z = (float)v[2];
+    a.w = (float)v[3];
+    *reinterpret_cast<f4*>(p) = a;
+}
+// v rounded to the storage type TV (identity for double)
+template <class TV>
+__device__ __forceinline__ double rnd(double v) {
+    return (double)(TV)v;
+}
+
+// w = z + p beta on the active right-hand-side columns (zero on the others)
+__device__ __forceinline__ void form_w(const double* zc, const double* pc, const double beta[NR][NR],
+                                       uint32_t act, double* w) {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        double sacc = 0.0;
-        for (uint32_t w = w0 + threadIdx.x; w < w1; w += 256) sacc += part[(uint64_t)w * NR + k];
-        const double pap = fhd::block_sum256(sacc, sh);
-        if (threadIdx.x == 0) {
-            const uint32_t col = inst * NR + k;
-            double a = 0.0;
-            if (!S.done[col]) {
-                if (fabs(pap) < 1e-30) {  // cg.rs:102-107, per column
-                    S.done[col] = 3;
-                    S.iters[col] = S.it[inst] + 1;
-                } else {
-                    a = S.rz[col] / pap;
-                }
-            }
-            S.alpha[col] = a;
-        }
+        double t = zc[k];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) t = fma(pc[m], beta[m][k], t);
+        w[k] = ((act >> k) & 1) ? t : 0.0;
     }
 }
 
-// x += alpha p, r -= alpha Ap, z = D^{-1} r; partials (r.r, r.z) per column.
-template <class T>
-__global__ __launch_bounds__(256) void k_b_update(const T* p0, const T* p1, const T* __restrict__ Ap,
-                                                  const T* __restrict__ dinv, T* __restrict__ x,
-                                                  T* __restrict__ r, T* __restrict__ z,
-                                                  BatchState S, uint32_t nrows,
-                                                  double* __restrict__ part) {
-    __shared__ double sh[4];
-    const uint32_t w = blockIdx.x, inst = S.wg_inst[w];
-    if (S.inst_done[inst] || S.need_rep[inst]) return;
-    const uint32_t j = S.it[inst] + 1;
-    const T* p = (j & 1) ? p1 : p0;
-    T al[NR];
+// ---------------------------------------------------------------- kernels
+// Per-instance state, written by the one-workgroup reduction kernels and read
+// by the tile kernels of the next launch.
+struct BbState {
+    uint32_t* act;     // [n_inst] right-hand-side columns still iterating
+    uint32_t* kept;    // [n_inst] directions kept by the last orthonormalisation
+    uint32_t* it;      // [n_inst] block iterations done
+    uint32_t* done;    // [n_inst] 1 when no column iterates any more
+    double* bnorm;     // [n_inst][4]
+    double* beta;      // [n_inst][16]  W = Z + P beta
+    double* L;         // [n_inst][16]  W^T A W = L L^T (kept directions)
+    double* alpha;     // [n_inst][16]  X += P alpha
+    uint32_t* iters;   // [n_inst][4]   iteration at which each column converged
+    uint8_t* status;   // [n_inst][4]   0 iterating, 1 converged, 2 not converged, 3 breakdown
+    const uint32_t* tile_inst;  // [n_tiles]
+    const uint32_t* inst_t0;    // [n_inst] first tile
+    const uint32_t* inst_t1;    // one past the last
+    uint32_t* active;  // [n_tiles] tiles of unfinished instances, then [n_inst] instances
+    uint32_t* nact;    // [2] active tiles, active instances
+    uint32_t n_tiles, n_inst;
+    int s;
+    double tol;
+    uint32_t max_iter;
+};
+
+constexpr int NA = NSYM + NR * NR;  // W^T A W (packed, from the product), W^T R (from W = Z + P beta)
+constexpr int NB = NR * NR + NR;    // update partials: Q^T Z, R^T R (diagonal)
+
+// R = B (X0 = 0), Z = D^{-1} R (the (z, p) row pairs zp = [z0..z3, p0..p3],
+// p = 0); tile partials R^T R and B^T B (diagonals) in the first 2 NR entries
+// of an NB record
+template <class TV>
+__global__ __launch_bounds__(BT) void k_bb_init(const double* __restrict__ b,
+                                                const double* __restrict__ dinv,
+                                                double* __restrict__ r, TV* __restrict__ zp,
+                                                double* __restrict__ partB, int s) {
+    __shared__ double sh[4 * 2 * NR];
+    const uint32_t t = blockIdx.x;
+    double acc[2 * NR];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) al[k] = (T)S.alpha[inst * NR + k];
-    const uint32_t row = w * 256 + threadIdx.x;
-    double rr[NR] = {0, 0, 0, 0}, rz[NR] = {0, 0, 0, 0};
-    if (row < nrows) {
-        const T d = dinv[row];
+    for (int k = 0; k < 2 * NR; ++k) acc[k] = 0.0;
+    for (int sub = 0; sub < TR / BT; ++sub) {
+        const uint64_t row = (uint64_t)t * TR + sub * BT + threadIdx.x;
+        double bv[NR], zv[NR];
+        ld4(b + row * NR, bv);
+        const double di = dinv[row];
+        const double zero[NR] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const uint64_t i = (uint64_t)row * NR + k;
-            x[i] = fma(al[k], p[i], x[i]);
-            const T ri = fma(-al[k], Ap[i], r[i]);
-            const T zi = ri * d;
-            r[i] = ri;
-            z[i] = zi;
-            rr[k] = (double)ri * (double)ri;
-            rz[k] = (double)ri * (double)zi;
+            if (k >= s) bv[k] = 0.0;
+            zv[k] = bv[k] * di;
+            acc[k] += bv[k] * bv[k];
         }
+        st4(r + row * NR, bv);
+        st4(zp + row * 2 * NR, zv);
+        st4(zp + row * 2 * NR + NR, zero);
     }
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const double a = fhd::block_sum256(rr[k], sh);
-        const double b = fhd::block_sum256(rz[k], sh);
-        if (threadIdx.x == 0) {
-            part[((uint64_t)w * NR + k) * 2] = a;
-            part[((uint64_t)w * NR + k) * 2 + 1] = b;
-        }
-    }
+    for (int k = 0; k < NR; ++k) acc[NR + k] = acc[k];
+    wg_sum<2 * NR>(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 2 * NR; ++k) partB[(uint64_t)t * NB + k] = acc[k];
 }
 
-// Per instance: residual test per column, beta, iteration count.  `init`
-// handles the first residual (cg.rs:80-84 initial test); `rep` a replaced
-// (true FP64) residual.
-__global__ __launch_bounds__(256) void k_b_red2(BatchState S, const double* __restrict__ part,
-                                                const double* __restrict__ bpart, int mode) {
-    __shared__ double sh[4];
-    const uint32_t inst = blockIdx.x;
-    // mode 0: after an update; 1: initial residual; 2: after a replacement
-    if (S.inst_done[inst]) return;
-    if (mode == 0 && S.need_rep[inst]) return;
-    if (mode == 2 && !S.need_rep[inst]) return;
-    const uint32_t w0 = S.wg_begin[inst], w1 = S.wg_end[inst];
-    const uint32_t it = S.it[inst] + (mode == 0 ? 1u : 0u);
-    double rr[NR], rz[NR], bb[NR];
-    for (int k = 0; k < NR; ++k) {
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (uint32_t w = w0 + threadIdx.x; w < w1; w += 256) {
-            a += part[((uint64_t)w * NR + k) * 2];
-            b += part[((uint64_t)w * NR + k) * 2 + 1];
-            if (mode == 1) c += bpart[(uint64_t)w * NR + k];
-        }
-        rr[k] = fhd::block_sum256(a, sh);
-        rz[k] = fhd::block_sum256(b, sh);
-        bb[k] = mode == 1 ? fhd::block_sum256(c, sh) : 0.0;
-    }
+// one workgroup per instance: the initial test per column (cg.rs:80-84; X0 = 0
+// so r0 = b and only ||b|| < tol stops it), beta = 0
+__global__ __launch_bounds__(BT) void k_bb_start(BbState S, const double* __restrict__ partB) {
+    __shared__ double sh[4 * NB];
+    const uint32_t i = blockIdx.x;
+    double red[NB];
+    reduce_slots<NB>(partB, S.inst_t0[i], S.inst_t1[i] - S.inst_t0[i], red, sh);
     if (threadIdx.x != 0) return;
-    // pass 1: stopping tests; in mixed mode a recursive residual that reached
-    // tol or dropped by delta asks for an FP64 replacement of the instance
-    int want_rep = 0;
-    bool conv[NR];
+    uint32_t act = 0;
     for (int k = 0; k < NR; ++k) {
-        const uint32_t col = inst * NR + k;
-        if (mode == 1) S.bnorm[col] = sqrt(bb[k]);
-        conv[k] = false;
-        if (S.done[col]) continue;
-        const double bn = S.bnorm[col];
-        const double res = sqrt(rr[k]);
-        const double rel = bn > 0.0 ? res / bn : res;
-        S.res[col] = res;
-        conv[k] = mode == 1 ? (res < S.tol || (bn > 0.0 && rel < S.tol)) : rel < S.tol;
-        if (mode == 0 && S.mixed && (conv[k] || res < S.delta * S.r_rep[col])) want_rep = 1;
+        const double bn = sqrt(red[NR + k]);
+        S.bnorm[i * NR + k] = bn;
+        const bool on = k < S.s && !(bn < S.tol);
+        if (on) act |= 1u << k;
+        S.status[i * NR + k] = on ? 0 : 1;
+        S.iters[i * NR + k] = 0;
     }
-    if (want_rep) {  // the recurrences resume from the replaced residual
-        S.it[inst] = it;  // iteration `it` was completed (x, r updated)
-        S.need_rep[inst] = 1;
-        return;
-    }
-    // pass 2: finish converged columns, advance the others
-    int all_done = 1;
-    for (int k = 0; k < NR; ++k) {
-        const uint32_t col = inst * NR + k;
-        if (S.done[col]) continue;
-        if (conv[k] && (mode != 0 || !S.mixed)) {
-            S.done[col] = 1;
-            S.iters[col] = it;
-            S.alpha[col] = 0.0;
-            S.beta[col] = 0.0;
-            continue;
-        }
-        if (mode == 0 && it >= S.max_iter) {
-            S.done[col] = 2;
-            S.iters[col] = it;
-            S.beta[col] = 0.0;
-            continue;
-        }
-        all_done = 0;
-        if (mode != 0) S.r_rep[col] = sqrt(rr[k]);
-        const bool restart = (mode == 1) || (mode == 2 && S.restart);
-        S.beta[col] = restart ? 0.0 : rz[k] / S.rz[col];
-        S.rz[col] = rz[k];
-    }
-    if (mode == 0) S.it[inst] = it;
-    if (mode == 2) S.need_rep[inst] = 0;
-    if (all_done) S.inst_done[inst] = 1;
+    for (int k = 0; k < NR * NR; ++k) S.beta[i * NR * NR + k] = 0.0;
+    S.act[i] = act;
+    S.kept[i] = 0;
+    S.it[i] = 0;
+    S.done[i] = act ? 0u : 1u;
 }
 
-// FP64 residual of the instances that asked for it (or all, at init):
-// t = b - A64 (x64 + x32); r = t, z = D^{-1} r; partials (t.t, r.z) and b.b.
-template <class T>
-__global__ __launch_bounds__(256) void k_b_residual(SellB<double> A, const double* __restrict__ b,
-                                                    const double* __restrict__ x64,
-                                                    const float* __restrict__ x32,
-                                                    const T* __restrict__ dinv, T* __restrict__ r,
-                                                    T* __restrict__ z, BatchState S, int only_rep,
-                                                    double* __restrict__ part,
-                                                    double* __restrict__ bpart) {
-    __shared__ double sh[4];
-    const uint32_t w = blockIdx.x, inst = S.wg_inst[w];
-    if (S.inst_done[inst]) return;
-    if (only_rep && !S.need_rep[inst]) return;
-    const uint32_t s = w * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    double tt[NR] = {0, 0, 0, 0}, rz[NR] = {0, 0, 0, 0}, bb[NR] = {0, 0, 0, 0};
-    if (s < A.nslices) {
-        const uint32_t row = s * SLICE + lane;
-        const uint32_t base = A.soff[s], wd = (A.soff[s + 1] - base) / SLICE;
-        const uint32_t len = row < A.nrows ? A.rlen[row] : 0;
-        double acc[NR] = {0, 0, 0, 0};
-        for (uint32_t q = 0; q < wd; ++q)
-            if (q < len) {
-                const uint64_t e = (uint64_t)base + (uint64_t)q * SLICE + lane;
-                const uint32_t c = A.cols[e];
-                const double v = A.vals[e];
+// W = Z + P beta (rounded to the storage type TV) for the rows of the active
+// tiles; tile partials W^T R into entries NSYM.. of the tile's NA record
+template <class TV>
+__global__ __launch_bounds__(BT) void k_bb_formw(const TV* __restrict__ zp,
+                                                 const double* __restrict__ r,
+                                                 TV* __restrict__ w, BbState S,
+                                                 double* __restrict__ partA) {
+    __shared__ double sh[4 * NR * NR];
+    const uint32_t na = S.nact[0];
+    for (uint32_t a = xcd_pos(); a < na; a += gridDim.x) {
+        const uint32_t t = S.active[a], inst = S.tile_inst[t];
+        if (S.done[inst]) continue;
+        const uint32_t act = S.act[inst];
+        double beta[NR][NR];
 #pragma unroll
-                for (int k = 0; k < NR; ++k) {
-                    double xv = x64[(uint64_t)c * NR + k];
-                    if (x32) xv = xv + (double)x32[(uint64_t)c * NR + k];
-                    acc[k] = fma(v, xv, acc[k]);
+        for (int m = 0; m < NR; ++m)
+#pragma unroll
+            for (int k = 0; k < NR; ++k) beta[m][k] = S.beta[(uint64_t)inst * NR * NR + m * NR + k];
+        double acc[NR * NR];
+#pragma unroll
+        for (int k = 0; k < NR * NR; ++k) acc[k] = 0.0;
+        for (int sub = 0; sub < TR / BT; ++sub) {
+            const uint64_t row = (uint64_t)t * TR + sub * BT + threadIdx.x;
+            double zo[NR], po[NR], wo[NR], rv[NR];
+            ld4(zp + row * 2 * NR, zo);
+            ld4(zp + row * 2 * NR + NR, po);
+            ld4(r + row * NR, rv);
+            form_w(zo, po, beta, act, wo);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) wo[k] = rnd<TV>(wo[k]);
+            st4(w + row * NR, wo);
+#pragma unroll
+            for (int a2 = 0; a2 < NR; ++a2)
+#pragma unroll
+                for (int c = 0; c < NR; ++c) acc[a2 * NR + c] += wo[a2] * rv[c];
+        }
+        wg_sum<NR * NR>(acc, sh);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int k = 0; k < NR * NR; ++k) partA[(uint64_t)t * NA + NSYM + k] = acc[k];
+    }
+}
+
+// Q_W = A W (FP64 accumulation of the stored W); tile partials W^T Q_W into
+// entries 0..NSYM of the tile's NA record.  Grid-stride over the active tiles.
+template <class TM, class TV>
+__global__ __launch_bounds__(BT) void k_bb_prod(SellB<TM> A, const TV* __restrict__ w,
+                                                double* __restrict__ q, BbState S,
+                                                double* __restrict__ partA) {
+    __shared__ double sh[4 * NSYM];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t na = S.nact[0];
+    for (uint32_t a = xcd_pos(); a < na; a += gridDim.x) {
+        const uint32_t t = S.active[a], inst = S.tile_inst[t];
+        if (S.done[inst]) continue;  // uniform per workgroup
+        double acc[NSYM];
+#pragma unroll
+        for (int k = 0; k < NSYM; ++k) acc[k] = 0.0;
+        for (int sub = 0; sub < TS / 4; ++sub) {
+            const uint32_t sl = t * TS + sub * 4 + wv;
+            const uint64_t row = (uint64_t)sl * SLICE + lane;
+            const uint32_t base = A.soff[sl], wd = (A.soff[sl + 1] - base) / SLICE;
+            const uint32_t* cp = A.cols + (uint64_t)base + lane;
+            const TM* vp = A.vals + (uint64_t)base + lane;
+            double qv[NR] = {0.0, 0.0, 0.0, 0.0};
+            // four stored entries per step, every load of a step issued before
+            // the arithmetic; SELL padding holds zero values and the row's own
+            // column, so it needs no length test (a step past the width
+            // re-reads the last entry with a zero value)
+            for (uint32_t e0 = 0; e0 < wd; e0 += 4) {
+                uint32_t c[4];
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t e = min(e0 + u, wd - 1);
+                    c[u] = cp[(uint64_t)e * SLICE];
+                    v[u] = e0 + u < wd ? (double)vp[(uint64_t)e * SLICE] : 0.0;
                 }
+                double wc[4][NR];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) ld4(w + (uint64_t)c[u] * NR, wc[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) qv[k] = fma(v[u], wc[u][k], qv[k]);
             }
-        if (row < A.nrows) {
-            const T d = dinv[row];
+            double wo[NR];
+            ld4(w + row * NR, wo);
+            st4(q + row * NR, qv);
+#pragma unroll
+            for (int a2 = 0; a2 < NR; ++a2)
+#pragma unroll
+                for (int c = a2; c < NR; ++c) acc[sym(a2, c)] += wo[a2] * qv[c];
+        }
+        wg_sum<NSYM>(acc, sh);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int k = 0; k < NSYM; ++k) partA[(uint64_t)t * NA + k] = acc[k];
+    }
+}
+
+// one workgroup per active instance: A-orthonormal directions P = W L^{-T}
+// (dependent ones dropped) and alpha = P^T R = L^{-1} (W^T R)
+__global__ __launch_bounds__(BT) void k_bb_red1(BbState S, const double* __restrict__ partA) {
+    __shared__ double sh[4 * NA];
+    const uint32_t ni = S.nact[1];
+    for (uint32_t a = blockIdx.x; a < ni; a += gridDim.x) {
+        const uint32_t i = S.active[S.n_tiles + a];
+        if (S.done[i]) continue;
+        double red[NA];
+        reduce_slots<NA>(partA, S.inst_t0[i], S.inst_t1[i] - S.inst_t0[i], red, sh);
+        if (threadIdx.x != 0) continue;
+        const uint32_t act = S.act[i];
+        double L[NR][NR];
+        const uint32_t kept = chol_drop4(red, act, L);
+        S.kept[i] = kept;
+        for (int m = 0; m < NR; ++m)
+            for (int k = 0; k < NR; ++k) S.L[(uint64_t)i * NR * NR + m * NR + k] = L[m][k];
+        for (int c = 0; c < NR; ++c) {
+            double u[NR], v[NR];
+            for (int m = 0; m < NR; ++m) u[m] = red[NSYM + m * NR + c];
+            lsolve_row(L, kept, u, v);
+            for (int m = 0; m < NR; ++m)
+                S.alpha[(uint64_t)i * NR * NR + m * NR + c] = ((act >> c) & 1) ? v[m] : 0.0;
+        }
+        if (!kept) {
+            // every direction vanished while columns are left: their residuals
+            // lie outside what W can reach
+            for (int k = 0; k < NR; ++k)
+                if ((act >> k) & 1) {
+                    S.status[i * NR + k] = 3;
+                    S.iters[i * NR + k] = S.it[i] + 1;
+                }
+            S.act[i] = 0;
+            S.done[i] = 1;
+        }
+    }
+}
+
+// P = W L^{-T}, Q = Q_W L^{-T} (FP64, from the stored W and Q_W = A W, so X
+// and R move by a consistent pair and the recursive residual tracks b - A x
+// to FP64 rounding whatever TV is); X += P alpha, R -= Q alpha, Z = D^{-1} R;
+// the (z, p) pair stored as TV; tile partials Q^T Z (the stored z) and R^T R
+template <class TV>
+__global__ __launch_bounds__(BT) void k_bb_upd(const TV* __restrict__ w,
+                                               const double* __restrict__ qw,
+                                               const double* __restrict__ dinv,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               TV* __restrict__ zp, BbState S,
+                                               double* __restrict__ partB) {
+    __shared__ double sh[4 * NB];
+    const uint32_t na = S.nact[0];
+    for (uint32_t a = xcd_pos(); a < na; a += gridDim.x) {
+        const uint32_t t = S.active[a], inst = S.tile_inst[t];
+        if (S.done[inst]) continue;
+        const uint32_t kept = S.kept[inst];
+        double L[NR][NR], alpha[NR][NR];
+#pragma unroll
+        for (int m = 0; m < NR; ++m)
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const uint64_t i = (uint64_t)row * NR + k;
-                const double t = b[i] - acc[k];
-                const T rv = (T)t, zv = rv * d;
-                r[i] = rv;
-                z[i] = zv;
-                tt[k] = t * t;
-                rz[k] = (double)rv * (double)zv;
-                bb[k] = b[i] * b[i];
+                L[m][k] = S.L[(uint64_t)inst * NR * NR + m * NR + k];
+                alpha[m][k] = S.alpha[(uint64_t)inst * NR * NR + m * NR + k];
             }
-        }
-    }
+        double acc[NB];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const double a = fhd::block_sum256(tt[k], sh);
-        const double c = fhd::block_sum256(rz[k], sh);
-        const double d = fhd::block_sum256(bb[k], sh);
-        if (threadIdx.x == 0) {
-            part[((uint64_t)w * NR + k) * 2] = a;
-            part[((uint64_t)w * NR + k) * 2 + 1] = c;
-            if (bpart) bpart[(uint64_t)w * NR + k] = d;
+        for (int k = 0; k < NB; ++k) acc[k] = 0.0;
+        for (int sub = 0; sub < TR / BT; ++sub) {
+        const uint64_t row = (uint64_t)t * TR + sub * BT + threadIdx.x;
+        double wv4[NR], qw4[NR], pv[NR], qv[NR], xv[NR], rv[NR], zv[NR];
+        ld4(w + row * NR, wv4);
+        ld4(qw + row * NR, qw4);
+        ld4(x + row * NR, xv);
+        ld4(r + row * NR, rv);
+        lsolve_row(L, kept, wv4, pv);
+        lsolve_row(L, kept, qw4, qv);
+        const double di = dinv[row];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            double dx = 0.0, dr = 0.0;
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                dx = fma(pv[m], alpha[m][k], dx);
+                dr = fma(qv[m], alpha[m][k], dr);
+            }
+            xv[k] += dx;
+            rv[k] -= dr;
+            zv[k] = rnd<TV>(rv[k] * di);
         }
+        st4(x + row * NR, xv);
+        st4(r + row * NR, rv);
+        st4(zp + row * 2 * NR, zv);
+        st4(zp + row * 2 * NR + NR, pv);
+#pragma unroll
+        for (int a2 = 0; a2 < NR; ++a2)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[a2 * NR + c] += qv[a2] * zv[c];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[NR * NR + k] += rv[k] * rv[k];
+        }
+        wg_sum<NB>(acc, sh);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) partB[(uint64_t)t * NB + k] = acc[k];
     }
 }
 
-__global__ void k_b_commit(double* __restrict__ x64, float* __restrict__ x32,
-                           const uint32_t* __restrict__ wg_inst, const uint8_t* __restrict__ need,
-                           uint32_t nrows, int all) {
-    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
-    if (row >= nrows) return;
-    if (!all && !need[wg_inst[blockIdx.x]]) return;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const uint64_t i = (uint64_t)row * NR + k;
-        x64[i] = x64[i] + (double)x32[i];
-        x32[i] = 0.0f;
+// one workgroup per active instance: per-column test on the recursive
+// residual (cg.rs:118-122; a converged column leaves the block, its X and R
+// stay), beta = -(P^T A P)^{-1} Q^T Z = -Q^T Z (P is A-orthonormal), the
+// iteration limit (cg.rs:139-148)
+__global__ __launch_bounds__(BT) void k_bb_red2(BbState S, const double* __restrict__ partB) {
+    __shared__ double sh[4 * NB];
+    const uint32_t ni = S.nact[1];
+    for (uint32_t a = blockIdx.x; a < ni; a += gridDim.x) {
+        const uint32_t i = S.active[S.n_tiles + a];
+        if (S.done[i]) continue;
+        double red[NB];
+        reduce_slots<NB>(partB, S.inst_t0[i], S.inst_t1[i] - S.inst_t0[i], red, sh);
+        if (threadIdx.x != 0) continue;
+        const uint32_t j = S.it[i] + 1;
+        uint32_t act = S.act[i];
+        const uint32_t kept = S.kept[i];
+        for (int k = 0; k < NR; ++k)
+            if ((act >> k) & 1) {
+                const double bn = S.bnorm[i * NR + k];
+                const double res = sqrt(red[NR * NR + k]);
+                const double rel = bn > 0.0 ? res / bn : res;
+                if (rel < S.tol) {
+                    S.status[i * NR + k] = 1;
+                    S.iters[i * NR + k] = j;
+                    act &= ~(1u << k);
+                }
+            }
+        if (act && j >= S.max_iter) {
+            for (int k = 0; k < NR; ++k)
+                if ((act >> k) & 1) {
+                    S.status[i * NR + k] = 2;
+                    S.iters[i * NR + k] = j;
+                }
+            act = 0;
+        }
+        for (int m = 0; m < NR; ++m)
+            for (int c = 0; c < NR; ++c)
+                S.beta[(uint64_t)i * NR * NR + m * NR + c] =
+                    (((kept >> m) & 1) && ((act >> c) & 1)) ? -red[m * NR + c] : 0.0;
+        S.act[i] = act;
+        S.it[i] = j;
+        if (!act) S.done[i] = 1;
     }
+}
+
+// the work list: tiles and instances not yet done, in order (one workgroup)
+__global__ __launch_bounds__(1024) void k_bb_compact(BbState S) {
+    __shared__ uint32_t cnt[1024];
+    __shared__ uint32_t tot;
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t n = pass == 0 ? S.n_tiles : S.n_inst;
+        const uint32_t per = (n + 1023) / 1024;
+        const uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+        uint32_t c = 0;
+        for (uint32_t k = b0; k < b1; ++k) {
+            const uint32_t inst = pass == 0 ? S.tile_inst[k] : k;
+            c += S.done[inst] ? 0u : 1u;
+        }
+        cnt[threadIdx.x] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < 1024; ++k) {
+                const uint32_t v = cnt[k];
+                cnt[k] = run;
+                run += v;
+            }
+            tot = run;
+        }
+        __syncthreads();
+        uint32_t o = cnt[threadIdx.x];
+        uint32_t* out = S.active + (pass == 0 ? 0 : S.n_tiles);
+        for (uint32_t k = b0; k < b1; ++k) {
+            const uint32_t inst = pass == 0 ? S.tile_inst[k] : k;
+            if (!S.done[inst]) out[o++] = k;
+        }
+        if (threadIdx.x == 0) S.nact[pass] = tot;
+        __syncthreads();
+    }
+}
+
+// tile partials of ||b - A64 x||^2 per column
+__global__ __launch_bounds__(BT) void k_bb_true_res(SellB<double> A, const double* __restrict__ b,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ part, int s) {
+    __shared__ double sh[4 * NR];
+    const uint32_t t = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc[NR] = {0.0, 0.0, 0.0, 0.0};
+    for (int sub = 0; sub < TS / 4; ++sub) {
+        const uint32_t sl = t * TS + sub * 4 + wv;
+        const uint64_t row = (uint64_t)sl * SLICE + lane;
+        const uint32_t base = A.soff[sl], wd = (A.soff[sl + 1] - base) / SLICE;
+        const uint32_t len = A.rlen[row];
+        double ax[NR] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t e = 0; e < wd && e < len; ++e) {
+            const uint64_t ix = (uint64_t)base + (uint64_t)e * SLICE + lane;
+            double xc[NR];
+            ld4(x + (uint64_t)A.cols[ix] * NR, xc);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) ax[k] = fma(A.vals[ix], xc[k], ax[k]);
+        }
+        double bv[NR];
+        ld4(b + row * NR, bv);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double tk = k < s ? bv[k] - ax[k] : 0.0;
+            acc[k] += tk * tk;
+        }
+    }
+    wg_sum<NR>(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) part[(uint64_t)t * NR + k] = acc[k];
 }
 
 // b = M f for NR columns (FP64).
@@ -388,15 +681,65 @@ static SellB<T> sellb(const fh_matrix* a, const T* vals) {
     return {p.slice_off.p, p.row_len.p, p.sell_cols.p, vals, p.nrows, p.nslices};
 }
 
+// Launch parameters of the tile kernels: workgroups per CU of their
+// grid-stride grid and block iterations per graph chunk, from
+// FH_BATCH_WGS_PER_CU / FH_BATCH_CHUNK, else the tuning file written by
+// fenics-rs_amd/tune.py (fh_tune.json beside the library, or the path in
+// FH_TUNE_FILE), else 16 and 16.
+struct BatchTune {
+    int wgs_per_cu = 16;
+    int chunk = 16;
+};
+
+// the integer after "key": in a small JSON text (-1 when absent)
+static int json_int(const std::string& t, const char* key) {
+    const std::string k = std::string("\"") + key + "\"";
+    const size_t p = t.find(k);
+    if (p == std::string::npos) return -1;
+    const size_t c = t.find(':', p + k.size());
+    if (c == std::string::npos) return -1;
+    return std::atoi(t.c_str() + c + 1);
+}
+
+static BatchTune batch_tune() {
+    BatchTune t;
+    std::string path;
+    if (const char* f = std::getenv("FH_TUNE_FILE")) {
+        path = f;
+    } else {
+        Dl_info info{};
+        if (dladdr((void*)&json_int, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t sl = path.rfind('/');
+            path = (sl == std::string::npos ? std::string(".") : path.substr(0, sl)) + "/fh_tune.json";
+        }
+    }
+    if (!path.empty()) {
+        if (FILE* fp = std::fopen(path.c_str(), "r")) {
+            std::string txt;
+            char buf[4096];
+            size_t n;
+            while ((n = std::fread(buf, 1, sizeof buf, fp)) > 0) txt.append(buf, n);
+            std::fclose(fp);
+            const int a = json_int(txt, "batched_wgs_per_cu"), b = json_int(txt, "batched_chunk");
+            if (a >= 1 && a <= 64) t.wgs_per_cu = a;
+            if (b >= 1 && b <= 256) t.chunk = b;
+        }
+    }
+    if (const char* e = std::getenv("FH_BATCH_WGS_PER_CU"))
+        t.wgs_per_cu = std::max(1, std::min(64, std::atoi(e)));
+    if (const char* e = std::getenv("FH_BATCH_CHUNK")) t.chunk = std::max(1, std::min(256, std::atoi(e)));
+    return t;
+}
+
 template <class T>
-static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_nodal, double tol,
-                       uint64_t max_iter, int precision, int dev, double* u_out,
-                       uint64_t* iters_out, fh_solve_stats* stats) {
-    constexpr bool MIX = std::is_same<T, float>::value;
+static int run_batched(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
+                       const double* f_nodal, double tol, uint64_t max_iter, int precision,
+                       int dev, double* u_out, uint64_t* iters_out, fh_solve_stats* stats) {
     DeviceCtx& d = device(dev);
     hipStream_t s = d.stream;
     const auto tw0 = std::chrono::steady_clock::now();
-    // ---- union mesh with per-instance padding to 256 rows
+    // ---- union mesh with per-instance padding to TR rows (one tile)
     std::vector<uint32_t> voff(n_inst + 1), coff(n_inst + 1), nvi(n_inst);
     uint64_t V = 0, Cn = 0;
     for (uint64_t i = 0; i < n_inst; ++i) {
@@ -404,7 +747,7 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_n
         voff[i] = (uint32_t)V;
         coff[i] = (uint32_t)Cn;
         nvi[i] = (uint32_t)((sizes[i] + 1) * (sizes[i] + 1));
-        V += (nvi[i] + 255) / 256 * 256;
+        V += (nvi[i] + TR - 1) / TR * TR;
         Cn += 2 * sizes[i] * sizes[i];
         FH_CHECK(V < 0x7fffffffull, "batch too large");
     }
@@ -441,16 +784,24 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_n
     std::unique_ptr<fh_matrix> Ac(
         apply_dirichlet(A.get(), dummy.p, fixed.size(), fixed.data(), zeros.data(), dummy2.p, s));
     A.reset();
-    // ---- right-hand sides b = M f (f given per real vertex)
+    // ---- right-hand sides b = M f (f given per real vertex, nrhs columns)
     const uint64_t nvec = V * NR;
-    DBuf<double> f(dev, nvec), b(dev, nvec), x64(dev, nvec);
+    DBuf<double> f(dev, nvec), b(dev, nvec);
     {
-        // each instance's rows are contiguous on both sides: one copy per
-        // instance into its padded range, the padding rows zero
         fill_f64(f.p, nvec, 0.0, s);
+        std::vector<double> hf;
         uint64_t src = 0;
         for (uint64_t i = 0; i < n_inst; ++i) {
-            h2d(f.p + (uint64_t)voff[i] * NR, f_nodal + src * NR, (size_t)nvi[i] * NR, s);
+            if (nrhs == (uint64_t)NR) {
+                h2d(f.p + (uint64_t)voff[i] * NR, f_nodal + src * NR, (size_t)nvi[i] * NR, s);
+            } else {
+                hf.assign((size_t)nvi[i] * NR, 0.0);
+                for (uint32_t r = 0; r < nvi[i]; ++r)
+                    for (uint64_t k = 0; k < nrhs; ++k)
+                        hf[(size_t)r * NR + k] = f_nodal[(src + r) * nrhs + k];
+                h2d(f.p + (uint64_t)voff[i] * NR, hf.data(), hf.size(), s);
+                FH_HIP(hipStreamSynchronize(s));
+            }
             src += nvi[i];
         }
         FH_HIP(hipStreamSynchronize(s));
@@ -459,175 +810,204 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, const double* f_n
     hipLaunchKernelGGL(k_spmm_f64, dim3(gslices), dim3(256), 0, s, sellb(M.get(), M->vals.p), f.p, b.p);
     hipLaunchKernelGGL(k_zero_fixed_rows, dim3(cdiv(V, 256)), dim3(256), 0, s, Ac->fixed.p, b.p,
                        (uint32_t)V);
+    FH_LAUNCH_CHECK("batched_rhs");
     M.reset();
-    // ---- workgroup -> instance tables (256 rows per workgroup)
-    const uint32_t nwg = (uint32_t)(V / 256);
-    std::vector<uint32_t> wg_inst(nwg), wg_begin(n_inst), wg_end(n_inst);
-    for (uint64_t i = 0; i < n_inst; ++i) {
-        wg_begin[i] = voff[i] / 256;
-        wg_end[i] = voff[i + 1] / 256;
-        for (uint32_t w = wg_begin[i]; w < wg_end[i]; ++w) wg_inst[w] = (uint32_t)i;
-    }
-    DBuf<uint32_t> dwi(dev, nwg), dwb(dev, n_inst), dwe(dev, n_inst), dit(dev, n_inst),
-        diters(dev, n_inst * NR);
-    h2d(dwi.p, wg_inst.data(), nwg, s);
-    h2d(dwb.p, wg_begin.data(), n_inst, s);
-    h2d(dwe.p, wg_end.data(), n_inst, s);
-    const uint64_t ncol = n_inst * NR;
-    DBuf<double> rz(dev, ncol), alpha(dev, ncol), beta(dev, ncol), bnorm(dev, ncol), rrep(dev, ncol),
-        res(dev, ncol);
-    DBuf<uint8_t> done(dev, ncol), idone(dev, n_inst), need(dev, n_inst);
-    fill_f64(beta.p, ncol, 0.0, s);
-    fill_f64(alpha.p, ncol, 0.0, s);
-    fill_f64(rz.p, ncol, 1.0, s);
-    fill_u8(done.p, ncol, 0, s);
-    fill_u8(idone.p, n_inst, 0, s);
-    fill_u8(need.p, n_inst, 0, s);
-    fill_u32(dit.p, n_inst, 0, s);
-    fill_u32(diters.p, ncol, 0, s);
-    BatchState S{rz.p, alpha.p, beta.p, bnorm.p, rrep.p, res.p, diters.p, done.p, dit.p, idone.p,
-                 need.p, dwb.p, dwe.p, dwi.p, tol,
-                 std::atof(std::getenv("FH_RR_DELTA") ? std::getenv("FH_RR_DELTA") : "0.1"),
-                 (uint32_t)std::min<uint64_t>(max_iter, 0xffffffffu), MIX ? 1u : 0u,
-                 precision == FH_PREC_MIXED_IR ? 1u : 0u};
-    // ---- Krylov vectors
+    f.reset();
+    DBuf<double> dinv(dev, V);
+    inv_diag_f64(Ac.get(), dinv.p, s);
+    constexpr bool MIX = std::is_same<T, float>::value;
     if (MIX) ensure_vals32(Ac.get());
-    DBuf<T> r(dev, nvec), z(dev, nvec), p0(dev, nvec), p1(dev, nvec), Ap(dev, nvec), dinv(dev, V);
-    DBuf<float> x32;
-    T* xk;
-    fill_f64(x64.p, nvec, 0.0, s);
+
+    // ---- tiles (TR rows) and the per-instance state
+    const uint32_t n_tiles = (uint32_t)(V / TR);
+    std::vector<uint32_t> tile_inst(n_tiles), t0(n_inst), t1(n_inst);
+    for (uint64_t i = 0; i < n_inst; ++i) {
+        t0[i] = voff[i] / TR;
+        t1[i] = voff[i + 1] / TR;
+        for (uint32_t t = t0[i]; t < t1[i]; ++t) tile_inst[t] = (uint32_t)i;
+    }
+    DBuf<uint32_t> dti(dev, n_tiles), dt0(dev, n_inst), dt1(dev, n_inst), act(dev, n_inst),
+        kept(dev, n_inst), itc(dev, n_inst), done(dev, n_inst), iters(dev, n_inst * NR),
+        active(dev, (size_t)n_tiles + n_inst), nact(dev, 2);
+    h2d(dti.p, tile_inst.data(), n_tiles, s);
+    h2d(dt0.p, t0.data(), n_inst, s);
+    h2d(dt1.p, t1.data(), n_inst, s);
+    DBuf<uint8_t> status(dev, n_inst * NR);
+    DBuf<double> bnorm(dev, n_inst * NR), beta(dev, n_inst * NR * NR), Lm(dev, n_inst * NR * NR),
+        alpha(dev, n_inst * NR * NR);
+    // Krylov vectors: x, r and q = A w in FP64; the gathered (z, p) pairs and w
+    // in TV (FP32 in the mixed modes)
+    using TV = T;
+    DBuf<double> x(dev, nvec), r(dev, nvec), q(dev, nvec), partA(dev, (size_t)n_tiles * NA),
+        partB(dev, (size_t)n_tiles * NB), part_t(dev, (size_t)n_tiles * NR);
+    DBuf<TV> zp(dev, 2 * nvec), wb(dev, nvec);
+    // padding rows and the rows of the unused columns stay zero in every vector
+    for (double* v : {x.p, r.p, q.p}) fill_f64(v, nvec, 0.0, s);
     if constexpr (MIX) {
-        x32.alloc(dev, nvec);
-        fill_f32(x32.p, nvec, 0.0f, s);
-        fill_f32((float*)p0.p, nvec, 0.0f, s);
-        fill_f32((float*)p1.p, nvec, 0.0f, s);
-        xk = (T*)x32.p;
+        fill_f32((float*)zp.p, 2 * nvec, 0.0f, s);
+        fill_f32((float*)wb.p, nvec, 0.0f, s);
     } else {
-        fill_f64((double*)p0.p, nvec, 0.0, s);
-        fill_f64((double*)p1.p, nvec, 0.0, s);
-        xk = (T*)x64.p;
+        fill_f64((double*)zp.p, 2 * nvec, 0.0, s);
+        fill_f64((double*)wb.p, nvec, 0.0, s);
     }
-    {
-        // Jacobi: 1/d of the condensed diagonal (cg.rs:52-64)
-        DBuf<double> dd(dev, V);
-        inv_diag_f64(Ac.get(), dd.p, s);
-        if constexpr (MIX) {
-            std::vector<double> h(V);
-            d2h(h.data(), dd.p, V, s);
-            FH_HIP(hipStreamSynchronize(s));
-            std::vector<float> hf(h.begin(), h.end());
-            h2d((float*)dinv.p, hf.data(), V, s);
-            FH_HIP(hipStreamSynchronize(s));
-        } else {
-            d2d((double*)dinv.p, dd.p, V, s);
-            FH_HIP(hipStreamSynchronize(s));
-        }
-    }
-    const SellB<double> A64 = sellb(Ac.get(), Ac->vals.p);
+    BbState S{act.p, kept.p, itc.p, done.p, bnorm.p, beta.p, Lm.p, alpha.p, iters.p, status.p,
+              dti.p, dt0.p, dt1.p, active.p, nact.p, n_tiles, (uint32_t)n_inst, (int)nrhs, tol,
+              (uint32_t)std::min<uint64_t>(std::max<uint64_t>(max_iter, 1), 0xffffffffu)};
     const SellB<T> AK = MIX ? sellb<T>(Ac.get(), (const T*)(const void*)Ac->vals32.p)
                             : sellb<T>(Ac.get(), (const T*)(const void*)Ac->vals.p);
-    DBuf<double> part1(dev, (size_t)nwg * NR), part2(dev, (size_t)nwg * NR * 2),
-        bpart(dev, (size_t)nwg * NR);
+    const SellB<double> A64 = sellb(Ac.get(), Ac->vals.p);
+    const BatchTune tn = batch_tune();
+    // a multiple of 8 (xcd_pos), at most the tile count rounded up
+    const unsigned gt = std::max(8u, std::min<unsigned>(cdiv(n_tiles, 8) * 8,
+                                                        (unsigned)(d.n_cu * tn.wgs_per_cu)) / 8 * 8);
+    const unsigned gi = std::max(1u, std::min<unsigned>((unsigned)n_inst, (unsigned)(d.n_cu * 4)));
     FH_HIP(hipStreamSynchronize(s));
 
     Timer tk(s);
-    hipLaunchKernelGGL(k_b_residual<T>, dim3(nwg), dim3(256), 0, s, A64, b.p, x64.p, (const float*)nullptr,
-                       dinv.p, r.p, z.p, S, 0, part2.p, bpart.p);
-    hipLaunchKernelGGL(k_b_red2, dim3(n_inst), dim3(256), 0, s, S, part2.p, bpart.p, 1);
+    hipLaunchKernelGGL(k_bb_init<TV>, dim3(n_tiles), dim3(BT), 0, s, b.p, dinv.p, r.p, zp.p,
+                       partB.p, (int)nrhs);
+    hipLaunchKernelGGL(k_bb_start, dim3((unsigned)n_inst), dim3(BT), 0, s, S, partB.p);
+    hipLaunchKernelGGL(k_bb_compact, dim3(1), dim3(1024), 0, s, S);
     FH_LAUNCH_CHECK("batched_init");
-    const int Mch = std::max(1, std::atoi(std::getenv("FH_CG_CHUNK") ? std::getenv("FH_CG_CHUNK") : "16"));
+    const int Mch = tn.chunk;
     auto chunk = [&]() {
-        for (int q = 0; q < Mch; ++q) {
-            hipLaunchKernelGGL(k_b_spmm<T>, dim3(nwg), dim3(256), 0, s, AK, z.p, p0.p, p1.p, Ap.p, S,
-                               part1.p);
-            hipLaunchKernelGGL(k_b_red1, dim3(n_inst), dim3(256), 0, s, S, part1.p);
-            hipLaunchKernelGGL(k_b_update<T>, dim3(nwg), dim3(256), 0, s, p0.p, p1.p, Ap.p, dinv.p, xk,
-                               r.p, z.p, S, (uint32_t)V, part2.p);
-            hipLaunchKernelGGL(k_b_red2, dim3(n_inst), dim3(256), 0, s, S, part2.p, bpart.p, 0);
+        for (int m = 0; m < Mch; ++m) {
+            hipLaunchKernelGGL(k_bb_formw<TV>, dim3(gt), dim3(BT), 0, s, (const TV*)zp.p,
+                               (const double*)r.p, wb.p, S, partA.p);
+            hipLaunchKernelGGL((k_bb_prod<T, TV>), dim3(gt), dim3(BT), 0, s, AK, (const TV*)wb.p,
+                               q.p, S, partA.p);
+            hipLaunchKernelGGL(k_bb_red1, dim3(gi), dim3(BT), 0, s, S, partA.p);
+            hipLaunchKernelGGL(k_bb_upd<TV>, dim3(gt), dim3(BT), 0, s, (const TV*)wb.p,
+                               (const double*)q.p, (const double*)dinv.p, x.p, r.p, zp.p, S,
+                               partB.p);
+            hipLaunchKernelGGL(k_bb_red2, dim3(gi), dim3(BT), 0, s, S, partB.p);
         }
+        hipLaunchKernelGGL(k_bb_compact, dim3(1), dim3(1024), 0, s, S);
     };
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    const bool use_graph = !std::getenv("FH_CG_GRAPH") || std::atoi(std::getenv("FH_CG_GRAPH")) != 0;
-    std::vector<uint8_t> hdone(n_inst), hneed(n_inst);
-    uint64_t launched = 0, nrep = 0;
-    for (;;) {
-        if (use_graph) {
-            if (!gexec) {
-                FH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                chunk();
-                FH_HIP(hipStreamEndCapture(s, &graph));
-                FH_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-            }
+    FH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    chunk();
+    FH_HIP(hipStreamEndCapture(s, &graph));
+    FH_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    // chunks are queued two ahead of the host's look at the work count: the
+    // host never waits on a chunk it has not already queued the next one behind
+    uint32_t* hn = nullptr;
+    FH_HIP(hipHostMalloc((void**)&hn, 2 * 2 * sizeof(uint32_t), hipHostMallocDefault));
+    hipEvent_t evs[2];
+    FH_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+    FH_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    uint64_t chunks = 0;
+    const uint64_t max_chunks = (uint64_t)S.max_iter / (uint64_t)Mch + 2;
+    bool finished = false;
+    for (uint64_t c = 0; c < max_chunks + 1 && !finished; ++c) {
+        if (c < max_chunks) {
             FH_HIP(hipGraphLaunch(gexec, s));
-        } else {
-            chunk();
-            FH_LAUNCH_CHECK("batched_chunk");
+            FH_HIP(hipMemcpyAsync(hn + 2 * (c & 1), nact.p, 2 * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            FH_HIP(hipEventRecord(evs[c & 1], s));
+            ++chunks;
         }
-        launched += Mch;
-        d2h(hdone.data(), idone.p, n_inst, s);
-        d2h(hneed.data(), need.p, n_inst, s);
-        FH_HIP(hipStreamSynchronize(s));
-        bool all = true, any_need = false;
-        for (uint64_t i = 0; i < n_inst; ++i) {
-            all = all && hdone[i];
-            any_need = any_need || (hneed[i] && !hdone[i]);
+        if (c >= 1) {  // the chunk before this one
+            FH_HIP(hipEventSynchronize(evs[(c - 1) & 1]));
+            if (hn[2 * ((c - 1) & 1) + 1] == 0) finished = true;
         }
-        if (all) break;
-        if (any_need) {
-            hipLaunchKernelGGL(k_b_residual<T>, dim3(nwg), dim3(256), 0, s, A64, b.p, x64.p, x32.p,
-                               dinv.p, r.p, z.p, S, 1, part2.p, (double*)nullptr);
-            hipLaunchKernelGGL(k_b_commit, dim3(nwg), dim3(256), 0, s, x64.p, x32.p, dwi.p, need.p,
-                               (uint32_t)V, 0);
-            hipLaunchKernelGGL(k_b_red2, dim3(n_inst), dim3(256), 0, s, S, part2.p, bpart.p, 2);
-            FH_LAUNCH_CHECK("batched_replacement");
-            ++nrep;
-        }
-        if (launched > max_iter + (nrep + 2) * (uint64_t)Mch + 64) break;  // safety net
     }
-    if constexpr (MIX)
-        hipLaunchKernelGGL(k_b_commit, dim3(nwg), dim3(256), 0, s, x64.p, x32.p, dwi.p, need.p,
-                           (uint32_t)V, 1);
+    FH_HIP(hipStreamSynchronize(s));
     const double kms = tk.stop_ms();
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(evs[0]);
+    (void)hipEventDestroy(evs[1]);
+    (void)hipHostFree(hn);
+    (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graph);
+    // ---- true FP64 residuals
+    hipLaunchKernelGGL(k_bb_true_res, dim3(n_tiles), dim3(BT), 0, s, A64, b.p, x.p, part_t.p,
+                       (int)nrhs);
+    FH_LAUNCH_CHECK("batched_true_res");
+    std::vector<double> hpt((size_t)n_tiles * NR), hbn(n_inst * NR), htr(n_inst * NR, 0.0);
+    d2h(hpt.data(), part_t.p, hpt.size(), s);
+    d2h(hbn.data(), bnorm.p, hbn.size(), s);
+    FH_HIP(hipStreamSynchronize(s));
+    for (uint64_t i = 0; i < n_inst; ++i)
+        for (int k = 0; k < NR; ++k) {
+            double tt = 0.0;
+            for (uint32_t t = t0[i]; t < t1[i]; ++t) tt += hpt[(size_t)t * NR + k];
+            const double bn = hbn[i * NR + k];
+            htr[i * NR + k] = bn > 0.0 ? std::sqrt(tt) / bn : std::sqrt(tt);
+        }
     // ---- outputs
-    std::vector<uint32_t> hit(ncol);
-    std::vector<uint8_t> hd(ncol);
+    std::vector<uint32_t> hit(n_inst * NR);
+    std::vector<uint8_t> hd(n_inst * NR);
     {
+        std::vector<double> hx;
         uint64_t dst = 0;
         for (uint64_t i = 0; i < n_inst; ++i) {  // the caller's rows of each instance
-            d2h(u_out + dst * NR, x64.p + (uint64_t)voff[i] * NR, (size_t)nvi[i] * NR, s);
+            if (nrhs == (uint64_t)NR) {
+                d2h(u_out + dst * NR, x.p + (uint64_t)voff[i] * NR, (size_t)nvi[i] * NR, s);
+            } else {
+                hx.resize((size_t)nvi[i] * NR);
+                d2h(hx.data(), x.p + (uint64_t)voff[i] * NR, hx.size(), s);
+                FH_HIP(hipStreamSynchronize(s));
+                for (uint32_t rr = 0; rr < nvi[i]; ++rr)
+                    for (uint64_t k = 0; k < nrhs; ++k)
+                        u_out[(dst + rr) * nrhs + k] = hx[(size_t)rr * NR + k];
+            }
             dst += nvi[i];
         }
     }
-    d2h(hit.data(), diters.p, ncol, s);
-    d2h(hd.data(), done.p, ncol, s);
+    d2h(hit.data(), iters.p, n_inst * NR, s);
+    d2h(hd.data(), status.p, n_inst * NR, s);
     FH_HIP(hipStreamSynchronize(s));
-    double dof_iters = 0.0;
-    uint64_t max_it = 0;
+    double dof_iters = 0.0, tr_max = 0.0;
+    uint64_t max_it = 0, inst_iters = 0;
     int code = FH_OK;
     for (uint64_t i = 0; i < n_inst; ++i) {
-        for (int k = 0; k < NR; ++k) {
+        uint64_t imax = 0;
+        for (uint64_t k = 0; k < nrhs; ++k) {
             const uint64_t col = i * NR + k;
-            if (iters_out) iters_out[col] = hit[col];
-            dof_iters += (double)nvi[i] * hit[col];
-            max_it = std::max<uint64_t>(max_it, hit[col]);
+            if (iters_out) iters_out[i * nrhs + k] = hit[col];
+            imax = std::max<uint64_t>(imax, hit[col]);
+            tr_max = std::max(tr_max, htr[col]);
             if (hd[col] == 3) code = FH_ERR_BREAKDOWN;
             else if (hd[col] != 1 && code == FH_OK) code = FH_ERR_NOT_CONVERGED;
         }
+        // every block iteration of the instance advances all nrhs columns
+        dof_iters += (double)nvi[i] * (double)nrhs * (double)imax;
+        inst_iters += imax;
+        max_it = std::max(max_it, imax);
     }
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->iterations = max_it;
-        stats->replacements = nrep;
+        stats->replacements = inst_iters;  // block iterations summed over instances
+        stats->true_rel_residual = tr_max;
         stats->kernel_ms = kms;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw0).count();
-        // DOF * iterations per second over all columns
+        // DOF (unknowns x columns) * iterations per second
         stats->gflops = dof_iters / (kms * 1e-3) / 1e9;
+        // one product pass: the matrix once for all columns, one read of the
+        // gathered w, the q store; vector rows are 4 columns of 8 bytes (x, r,
+        // q) or of sizeof(T) (z, p, w)
+        const double vb = 4.0 * sizeof(T);
         stats->spmv_bytes = (double)Ac->pat->nnzb * (sizeof(T) + 4.0) + 4.0 * (V + 1) +
-                            2.0 * (double)nvec * sizeof(T);
+                            (vb + 32.0) * (double)V;
+        // one block iteration: W = Z + P beta (reads the (z, p) pair and r,
+        // writes w), the product pass, and the update (reads w, q, x, r and
+        // D^{-1}; writes x, r and the (z, p) pair)
+        stats->iter_bytes = stats->spmv_bytes + (3.0 * vb + 32.0) * (double)V +
+                            (vb + 32.0 * 3.0 + 8.0 + 32.0 * 2.0 + 2.0 * vb) * (double)V;
+        stats->effective_bw_gbps = 0.0;
+        if (kms > 0.0) {
+            double bytes = 0.0;
+            for (uint64_t i = 0; i < n_inst; ++i) {
+                uint64_t imax = 0;
+                for (uint64_t k = 0; k < nrhs; ++k) imax = std::max<uint64_t>(imax, hit[i * NR + k]);
+                bytes += (double)imax * stats->iter_bytes * (double)(voff[i + 1] - voff[i]) / (double)V;
+            }
+            stats->effective_bw_gbps = bytes / (kms * 1e-3) / 1e9;
+        }
+        stats->spmv_timed = chunks;
         stats->device = dev;
         stats->precision = precision;
         stats->status = code;
@@ -646,16 +1026,17 @@ extern "C" int fh_batched_poisson_block_cg(uint64_t n_inst, const uint64_t* size
     int code = FH_OK;
     const int g = guard([&] {
         FH_CHECK(sizes && f_nodal && u_out && n_inst > 0, "null argument");
-        FH_CHECK(nrhs == (uint64_t)NR, "this build solves 4 right-hand sides per instance");
+        FH_CHECK(nrhs >= 1 && nrhs <= (uint64_t)NR, "the block size must be 1 to 4 right-hand sides");
+        FH_CHECK(precision >= 0 && precision <= 3, "unknown precision");
         const int dev = resolve_device(device_idx);
-        if (precision == FH_PREC_FP64)
-            code = run_batched<double>(n_inst, sizes, f_nodal, tol, max_iter, precision, dev, u_out,
-                                       iters_out, stats);
+        if (precision == FH_PREC_FP64 || precision == FH_PREC_FP64_TRUE)
+            code = run_batched<double>(n_inst, sizes, nrhs, f_nodal, tol, max_iter, precision, dev,
+                                       u_out, iters_out, stats);
         else
-            code = run_batched<float>(n_inst, sizes, f_nodal, tol, max_iter, precision, dev, u_out,
-                                      iters_out, stats);
+            code = run_batched<float>(n_inst, sizes, nrhs, f_nodal, tol, max_iter, precision, dev,
+                                      u_out, iters_out, stats);
         if (code == FH_ERR_NOT_CONVERGED) set_last_error("a batched column failed to converge");
-        if (code == FH_ERR_BREAKDOWN) set_last_error("breakdown in a batched column");
+        if (code == FH_ERR_BREAKDOWN) set_last_error("breakdown in a batched block");
     });
     return g != FH_OK ? g : code;
 }

@@ -870,20 +870,25 @@ def cell_bilinear_k0(mesh: Mesh, nu, w, u) -> np.ndarray:
 
 def batched_poisson_block_cg(sizes, f_nodal, tol=1e-10, max_iter=20000, precision=FP64,
                              device=-1):
-    """Many unit_square(N_i, N_i) Poisson problems with 4 right-hand sides each
-    (b = M f from nodal loads f: list of [(N_i+1)^2, 4] arrays).  Returns
-    (list of solutions, iterations [n_inst, 4], SolveStats)."""
+    """Many unit_square(N_i, N_i) Poisson problems solved by block CG in one
+    persistent launch, with nrhs = 1..4 right-hand sides each (b = M f from
+    nodal loads f: list of [(N_i+1)^2, nrhs] arrays, or [(N_i+1)^2] for one).
+    Returns (list of solutions of f's shape, iterations [n_inst, nrhs] (the
+    block iteration at which each column converged), SolveStats)."""
     sizes = _u64(sizes)
-    f = _f64(np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1, 4) for a in f_nodal]))
+    arrs = [np.asarray(a, dtype=np.float64) for a in f_nodal]
+    squeeze = arrs[0].ndim == 1
+    nrhs = 1 if squeeze else arrs[0].shape[1]
+    f = _f64(np.concatenate([a.reshape(-1, nrhs) for a in arrs]))
     u = np.empty_like(f)
-    its = np.empty((len(sizes), 4), np.uint64)
+    its = np.empty((len(sizes), nrhs), np.uint64)
     st = SolveStats()
-    _check(lib().fh_batched_poisson_block_cg(len(sizes), _p(sizes), 4, _p(f), tol, max_iter,
+    _check(lib().fh_batched_poisson_block_cg(len(sizes), _p(sizes), nrhs, _p(f), tol, max_iter,
                                              precision, device, _p(u), _p(its), C.byref(st)))
     out, o = [], 0
     for N in sizes:
         k = (int(N) + 1) ** 2
-        out.append(u[o:o + k])
+        out.append(u[o:o + k, 0].copy() if squeeze else u[o:o + k])
         o += k
     return out, its, st
 

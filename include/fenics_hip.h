@@ -314,11 +314,20 @@ int fh_vertex_von_mises(fh_mesh* m, const double* vm_cell, double* out);
 /* ------------------------------------------------ batched multi-RHS solve
  * New (the reference solves one system per solve_poisson_2d call,
  * poisson_solver.rs:26): n_inst independent unit_square(N_i, N_i) Poisson P1
- * problems (generators.rs:24-55, u = 0 on the boundary) with nrhs (= 4)
+ * problems (generators.rs:24-55, u = 0 on the boundary) with nrhs (1..4)
  * right-hand sides each, b = M f (M the P1 mass matrix, mass.rs:10-44; f the
- * nodal loads f_nodal[sum_i (N_i+1)^2][nrhs]).  Every column is solved by
- * Jacobi-PCG to ||r||/||b|| < tol (true FP64 residual in the mixed modes);
- * u_out has f_nodal's layout, iters_out[n_inst][nrhs] the per-column counts. */
+ * nodal loads f_nodal[sum_i (N_i+1)^2][nrhs]).  Every instance is solved by
+ * Jacobi-preconditioned block CG (O'Leary; nrhs = 1 is cg.rs:40-149) in one
+ * persistent launch over a device-side work queue, each column to
+ * ||r||/||b|| < tol on its recursive residual; converged columns leave the
+ * block.  FH_PREC_FP64 reads FP64 matrix values, the mixed modes an FP32 copy
+ * (exact for this operator); vectors and reductions are FP64.  u_out has
+ * f_nodal's layout, iters_out[n_inst][nrhs] the block iteration at which each
+ * column converged.  stats: iterations = the largest count, replacements =
+ * block iterations summed over instances, true_rel_residual = the largest
+ * true FP64 residual of any column, gflops = unknowns x columns x iterations
+ * per second / 1e9, spmv_bytes / iter_bytes = algorithmic bytes of one
+ * product pass / block iteration over the whole batch.                       */
 int fh_batched_poisson_block_cg(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
                                 const double* f_nodal, double tol, uint64_t max_iter,
                                 int precision, int device, double* u_out,

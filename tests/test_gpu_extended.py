@@ -9,6 +9,7 @@ with central finite differences to 1e-6 relative (the required bar) and satisfy
 sum_c E_c dJ/dE_c = -J to 1e-9; batched FP64 columns agree with the oracle's
 CG to 1e-8 relative, mixed-precision ones to 1e-7."""
 import numpy as np
+import scipy.sparse.linalg as spla
 import pytest
 
 import fenics_hip as fh
@@ -108,6 +109,8 @@ def test_compliance_adjoint_torch_vs_fd():
 
 
 def oracle_batched_column(N, f):
+    """One column through the oracle: the condensed system (cg.rs's CG
+    iteration count at 1e-10) and its direct solution."""
     om = O.unit_square(N, N)
     K = O.assemble_stiffness(om)
     M = O.assemble_mass_2d(om)
@@ -115,48 +118,81 @@ def oracle_batched_column(N, f):
     bd = O.boundary_dofs(om)
     Kb, bb = O.apply_dirichlet_bc(K, b, bd, np.zeros(len(bd)))
     st, x, it, res = O.solve_cg(Kb, bb, None, 1e-10, 20000)
-    return x, it
+    xd = spla.spsolve(Kb.to_scipy().tocsc(), bb)
+    return xd, it
 
 
 @pytest.mark.parametrize("prec", [fh.FP64, fh.MIXED])
 def test_batched_multi_rhs_vs_oracle(prec):
+    """Block CG over four columns per instance: each column against the
+    oracle's direct solution (error <= kappa * tol, kappa < 1e3 here), and no
+    column takes more iterations than cg.rs's independent CG on it (the block
+    shares one Krylov space); over all columns it takes fewer."""
     rng = np.random.default_rng(21)
     sizes = [3, 8, 17, 30, 41]
     fs = [rng.uniform(-1, 1, ((N + 1) ** 2, 4)) for N in sizes]
     us, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, precision=prec)
+    assert st.status == fh.FH_OK and st.true_rel_residual < 2e-10
+    tot_block = tot_cg = 0
     for i, N in enumerate(sizes):
         for k in range(4):
             xo, ito = oracle_batched_column(N, fs[i][:, k])
             rel = np.linalg.norm(us[i][:, k] - xo) / max(np.linalg.norm(xo), 1e-300)
-            assert rel < (1e-8 if prec == fh.FP64 else 1e-7), (N, k, rel)
-            if prec == fh.FP64:
-                assert abs(int(its[i, k]) - ito) <= 2
+            assert rel < 1e-7, (N, k, rel)
+            assert int(its[i, k]) <= ito + 2, (N, k, int(its[i, k]), ito)
+            tot_block += int(its[i, k])
+            tot_cg += ito
+    assert tot_block < tot_cg
 
 
 def test_batched_single_rhs_block_equals_cg():
-    """A batch of one instance reproduces the single-system solver column by column."""
+    """With one right-hand side, block CG is cg.rs's CG: a batch of one
+    instance reproduces the single-system solver."""
     N = 24
     rng = np.random.default_rng(5)
-    f = rng.uniform(-1, 1, ((N + 1) ** 2, 4))
+    f = rng.uniform(-1, 1, (N + 1) ** 2)
     us, its, st = fh.batched_poisson_block_cg([N], [f], tol=1e-12, precision=fh.FP64)
     m = fh.unit_square(N, N)
     K = fh.assemble_stiffness(m)
     M = fh.assemble_mass(m)
     bd = m.boundary_dofs()
-    for k in range(4):
-        Kb, bb = fh.apply_dirichlet_bc(K, M.spmv(f[:, k]), fh.DirichletBC(bd, np.zeros(len(bd))))
-        x, s1 = Kb.solve(bb, tol=1e-12, max_iter=5000)
-        assert np.linalg.norm(us[0][:, k] - x) / np.linalg.norm(x) < 1e-10
-        assert abs(int(its[0, k]) - int(s1.iterations)) <= 1
+    Kb, bb = fh.apply_dirichlet_bc(K, M.spmv(f), fh.DirichletBC(bd, np.zeros(len(bd))))
+    x, s1 = Kb.solve(bb, tol=1e-12, max_iter=5000)
+    assert np.linalg.norm(us[0] - x) / np.linalg.norm(x) < 1e-10
+    assert abs(int(its[0, 0]) - int(s1.iterations)) <= 1
+
+
+@pytest.mark.parametrize("nrhs", [2, 3])
+def test_batched_partial_blocks_and_determinism(nrhs):
+    """Blocks of two and three columns, and a zero column (converged at
+    iteration 0, cg.rs:80-84); two runs are bitwise equal."""
+    rng = np.random.default_rng(30 + nrhs)
+    sizes = [5, 33, 70]
+    fs = [rng.uniform(-1, 1, ((N + 1) ** 2, nrhs)) for N in sizes]
+    fs[1][:, 0] = 0.0
+    us, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, precision=fh.FP64)
+    us2, its2, _ = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, precision=fh.FP64)
+    assert int(its[1, 0]) == 0 and not us[1][:, 0].any()
+    for a, b in zip(us, us2):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert np.array_equal(its, its2)
+    for i, N in enumerate(sizes):
+        for k in range(nrhs):
+            if not fs[i][:, k].any():
+                continue
+            xo, ito = oracle_batched_column(N, fs[i][:, k])
+            assert np.linalg.norm(us[i][:, k] - xo) / np.linalg.norm(xo) < 1e-7
+            assert int(its[i, k]) <= ito + 2
 
 
 def test_c4_full_size_batched_fp64_and_mixed():
     """The full batched workload: 256 unit_square(N_i, N_i) instances with
-    N_i ~ U{32..512} (seeded), four nodal loads each (b = M f), tol 1e-10.
-    Every column converges in both precisions; the true residual of sampled
-    instances (the two smallest, the two largest, four random ones),
-    recomputed instance by instance with the single-system path, is within 2x
-    of tol; mixed and FP64 solutions agree to 1e-8 relative."""
+    N_i ~ U{32..512} (seeded), four nodal loads each (b = M f), tol 1e-10, in
+    one persistent launch.  Every column converges in both precisions; the
+    true residual of sampled instances (the two smallest, the two largest, four
+    random ones), recomputed instance by instance with the single-system path,
+    is within 2x of tol; mixed and FP64 solutions agree to 1e-8 relative; the
+    block takes fewer iterations than the single-system CG on each column."""
     rng = np.random.default_rng(2024)
     sizes = rng.integers(32, 513, 256)
     fs = [rng.uniform(-1, 1, ((int(N) + 1) ** 2, 4)) for N in sizes]
@@ -165,9 +201,11 @@ def test_c4_full_size_batched_fp64_and_mixed():
     for prec in (fh.FP64, fh.MIXED):
         us, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, max_iter=40000,
                                                   precision=prec)
-        print(f"C4 precision {prec}: {st.wall_ms:.1f} ms, column iterations {its.min()}-{its.max()}",
-              flush=True)
+        print(f"C4 precision {prec}: {st.kernel_ms:.1f} ms kernel, {st.wall_ms:.1f} ms wall, "
+              f"column iterations {its.min()}-{its.max()}, {st.gflops:.2f} GDOF.iter/s, "
+              f"true rel res {st.true_rel_residual:.2e}", flush=True)
         assert its.min() > 0 and st.status == fh.FH_OK
+        assert st.true_rel_residual < 2e-10
         sols[prec] = us
         for i in picks:
             N = int(sizes[i])
@@ -179,6 +217,9 @@ def test_c4_full_size_batched_fp64_and_mixed():
                                                fh.DirichletBC(bd, np.zeros(len(bd))))
                 rel = np.linalg.norm(bb - Kb.spmv(us[i][:, k])) / np.linalg.norm(bb)
                 assert rel < 2e-10, (prec, i, N, k, rel)
+                if prec == fh.FP64 and k == 0:
+                    _, s1 = Kb.solve(bb, tol=1e-10, max_iter=40000, precision=fh.FP64)
+                    assert int(its[i].max()) < int(s1.iterations), (N, its[i], s1.iterations)
     for a, b in zip(sols[fh.FP64], sols[fh.MIXED]):
         assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-8
 
