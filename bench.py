@@ -186,6 +186,9 @@ def main():
     ap.add_argument("--nx", type=int, default=200, help="cantilever cells along x (ny=nz=nx/5)")
     ap.add_argument("--ceiling", action="store_true",
                     help="only measure the device bandwidth ceilings and print them as JSON")
+    ap.add_argument("--batched-once", choices=("fp64", "mixed"), default=None,
+                    help="profiling run: one batched block-CG solve of the 256-instance "
+                         "workload in this precision; prints its stats, no JSON")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
                          "and of the FP32-value SpMV on the cantilever, then a 1 GiB device "
@@ -209,6 +212,17 @@ def main():
     torch.cuda.set_device(dev)
     if args.ceiling:
         print(json.dumps({"device": gpus[dev].name, "ceilings": ceilings()}))
+        return
+    if args.batched_once:
+        rng = np.random.default_rng(2024)
+        sizes = rng.integers(32, 513, 256)
+        fs = [rng.uniform(-1, 1, ((int(N) + 1) ** 2, 4)) for N in sizes]
+        prec = fh.MIXED if args.batched_once == "mixed" else fh.FP64
+        _, its, st = fh.batched_poisson_block_cg(sizes, fs, tol=1e-10, max_iter=40000,
+                                                 precision=prec)
+        log(f"batched {args.batched_once}: kernel {st.kernel_ms:.1f} ms, wall {st.wall_ms:.1f} ms, "
+            f"block iterations <= {its.max()}, summed {st.replacements}, chunks {st.spmv_timed}, "
+            f"{st.gflops:.2f} GDOF.iter/s")
         return
 
     nx = args.nx
