@@ -173,6 +173,28 @@ def batched_c4(fh, np):
     return out
 
 
+def assembly_once(fh, np, torch, dev, nx, reps):
+    """REPS assemblies of each assembly workload, timed around a device sync."""
+    mesh = fh.box_beam(10.0, 2.0, 2.0, nx, nx // 5, nx // 5, device=dev)
+    mat = fh.ElasticMaterial(210e9, 0.3)
+    ec = np.random.default_rng(3).uniform(0.5e9, 2e9, mesh.nc)
+    cube = fh.unit_cube(64, 64, 64, device=dev)
+    runs = [("elasticity", lambda: fh.assemble_elasticity_stiffness_3d(mesh, mat)),
+            ("elasticity E_cell", lambda: fh.assemble_elasticity_stiffness_3d(mesh, mat, ec)),
+            ("p2 valu", lambda: fh.assemble_p2_stiffness(cube, 0)),
+            ("p2 mfma", lambda: fh.assemble_p2_stiffness(cube, 1))]
+    for name, fn in runs:
+        a = fn()  # patterns, adjacency and P2 numbering built outside the timing
+        torch.cuda.synchronize()
+        del a
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            a = fn()
+            torch.cuda.synchronize()
+            log(f"assembly {name}: {1e3 * (time.perf_counter() - t0):.2f} ms")
+            del a
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +211,10 @@ def main():
     ap.add_argument("--batched-once", choices=("fp64", "mixed"), default=None,
                     help="profiling run: one batched block-CG solve of the 256-instance "
                          "workload in this precision; prints its stats, no JSON")
+    ap.add_argument("--assembly-once", type=int, default=0, metavar="REPS",
+                    help="profiling run: REPS assemblies of the cantilever elasticity stiffness "
+                         "(uniform E, then per-cell E) and of the P2 Poisson unit_cube(64^3) "
+                         "stiffness (FP64 VALU, then FP32 MFMA); prints the wall times, no JSON")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
                          "and of the FP32-value SpMV on the cantilever, then a 1 GiB device "
@@ -223,6 +249,10 @@ def main():
         log(f"batched {args.batched_once}: kernel {st.kernel_ms:.1f} ms, wall {st.wall_ms:.1f} ms, "
             f"block iterations <= {its.max()}, summed {st.replacements}, chunks {st.spmv_timed}, "
             f"{st.gflops:.2f} GDOF.iter/s")
+        return
+
+    if args.assembly_once:
+        assembly_once(fh, np, torch, dev, args.nx, args.assembly_once)
         return
 
     nx = args.nx

@@ -14,6 +14,7 @@
 //     bit-identical too.
 // Values are written straight into the SELL-64 layout the CG kernels read.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -118,42 +119,213 @@ __global__ __launch_bounds__(256) void k_asm_p1_tet(
     }
 }
 
-// elasticity.rs:86-146: the 3x3 blocks of row vertex v (local index a) against
-// every vertex b of each incident cell; E per cell when Ecell != nullptr.
-__global__ __launch_bounds__(256) void k_asm_elasticity(
-    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
-    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv, double E,
-    double nu, const double* __restrict__ Ecell, double* __restrict__ vals) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
-    const RowCtx rc = row_ctx(soff, rlen, v);
-    init_row<9>(vals, rc);
-    double lambda = lame_lambda(E, nu), mu = lame_mu(E, nu);
-    for (uint32_t e = aptr[v]; e < aptr[v + 1]; ++e) {
-        const uint32_t c = aent[e] / 4, a = aent[e] % 4;
-        double x[4][3];
-        uint32_t t[4];
-        load_cell<3, 4>(V, C, c, x, t);
-        Map3 m;
-        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
-        if (Ecell) {
-            const double ec = Ecell[c];
-            lambda = lame_lambda(ec, nu);
-            mu = lame_mu(ec, nu);
-        }
-        const double vol = (1.0 / 6.0) * fabs(m.det);
+// elasticity.rs:86-146 in two kernels.
+//
+// 1. k_elas_cell_geom, one thread per cell: the cell's P1 gradients
+//    g[0..3][0..2], its volume and, with a per-cell E, its Lame constants,
+//    into a 128-byte record (vol < 0 marks a cell the reference skips,
+//    |det J| < 1e-14).  The geometry is computed once per cell instead of
+//    once per (row, cell) pair.
+// 2. k_asm_elasticity_rows: the 3x3 blocks of each row vertex v (local index
+//    a in the cell) against every vertex b of each incident cell.  One wave
+//    takes 16 rows of a SELL slice, four lanes per row, lane q the block of
+//    cell vertex b = q, so a wave's accumulator is 16 rows wide.  The blocks
+//    are summed in LDS (elas_acc_index: each half-wave's 8 rows own four
+//    consecutive doubles of every 32-double line, picked by k mod 4, so two
+//    lanes of a half-wave meet in a bank only when two blocks of one cell are
+//    congruent mod 4), and written to the SELL values once, with 16-byte
+//    stores of the component pairs.  Every row still walks its incident cells
+//    in ascending order, so every value is the oracle's sum, bit for bit.
+//    The accumulator holds `wp` blocks per row; a slice wider than that is
+//    done in passes over ranges of wp blocks.  Work unit u = 4 * slice +
+//    quarter; block b of the grid takes unit (b % 8) * per + b / 8, so the
+//    units one XCD runs are consecutive and share cells in that XCD's L2.
+constexpr int ELAS_GS = 16;  // doubles per cell record
+
+template <bool EC>
+__global__ __launch_bounds__(256) void k_elas_cell_geom(const double* __restrict__ V,
+                                                        const uint32_t* __restrict__ C,
+                                                        uint32_t nc, double nu,
+                                                        const double* __restrict__ Ecell,
+                                                        double* __restrict__ G) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    double o[ELAS_GS];
+#pragma unroll
+    for (int i = 0; i < ELAS_GS; ++i) o[i] = 0.0;
+    double x[4][3];
+    uint32_t t[4];
+    Map3 m;
+    if (c >= nc) {
+        // no cell: the record is never stored
+    } else if (load_cell<3, 4>(V, C, c, x, t), affine3(x[0], x[1], x[2], x[3], m)) {
         double g[4][3];
         p1_tet_grads(m, g);
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t k = slots[(uint64_t)e * 4 + b];
 #pragma unroll
-            for (int al = 0; al < 3; ++al)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int be = 0; be < 3; ++be)
-                    vals[vidx(rc.soff, k, al * 3 + be, rc.lane, 9)] +=
-                        elas_entry(g[a], g[b], al, be, lambda, mu, vol);
+            for (int i = 0; i < 3; ++i) o[3 * b + i] = g[b][i];
+        o[12] = (1.0 / 6.0) * fabs(m.det);
+        if (EC) {
+            const double ec = Ecell[c];
+            o[13] = lame_lambda(ec, nu);
+            o[14] = lame_mu(ec, nu);
         }
+    } else {
+        o[12] = -1.0;
+    }
+    // the block's records are one contiguous range: stage them in LDS and
+    // store them as consecutive 16-byte pieces
+    // (a thread's 8 pieces at stride 9, so a wave's stores spread over the banks)
+    constexpr uint32_t NP = ELAS_GS / 2, SP = NP + 1;
+    __shared__ d2 stage[256 * SP];
+#pragma unroll
+    for (uint32_t i = 0; i < NP; ++i) {
+        d2 v;
+        v.x = o[2 * i];
+        v.y = o[2 * i + 1];
+        stage[threadIdx.x * SP + i] = v;
+    }
+    __syncthreads();
+    const uint32_t c0 = blockIdx.x * 256;
+    const uint32_t npieces = min(256u, nc - c0) * NP;
+    d2* out = reinterpret_cast<d2*>(G + (uint64_t)c0 * ELAS_GS);
+    for (uint32_t i = threadIdx.x; i < npieces; i += 256) out[i] = stage[(i / NP) * SP + i % NP];
+}
+
+constexpr int ELAS_RPW = 16;  // rows per wave of k_asm_elasticity_rows
+// LDS position of component j of block kk (counted from the pass's first
+// block) of row r (0-15): line j + 9 (kk / 4) holds, for each half of the
+// rows, 8 rows x 4 blocks
+__device__ __forceinline__ uint32_t elas_acc_index(uint32_t r, uint32_t kk, uint32_t j) {
+    return ((j + 9 * (kk / 4)) * 2 + r / 8) * 32 + (r % 8) * 4 + kk % 4;
+}
+__host__ __device__ inline uint32_t elas_acc_size(uint32_t nk) { return 9 * ((nk + 3) / 4) * 64; }
+
+// what a lane needs of one incident cell: the row's gradient, the column
+// vertex's gradient, vol, the Lame constants and the column's block slot,
+// plus the index entry and slot of the cell this record takes after it
+struct ElasRec {
+    double ga[3], gb[3], vol, lam, mu;
+    uint32_t k, nen, nk;
+};
+template <bool EC>
+__device__ __forceinline__ void rec_load(const double* __restrict__ G, uint32_t en, uint32_t kq,
+                                         uint32_t q, double lambda0, double mu0, ElasRec& r) {
+    const double* gp = G + (uint64_t)(en / 4) * ELAS_GS;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        r.ga[i] = gp[3 * (en % 4) + i];
+        r.gb[i] = gp[3 * q + i];
+    }
+    r.vol = gp[12];
+    r.lam = EC ? gp[13] : lambda0;
+    r.mu = EC ? gp[14] : mu0;
+    r.k = kq;
+}
+// first fill: data of cell e, index of cell e + 3 (positions past the row's
+// last cell re-read the last one and are never used)
+template <bool EC>
+__device__ __forceinline__ void rec_start(const double* __restrict__ G,
+                                          const uint32_t* __restrict__ aent,
+                                          const uint8_t* __restrict__ slots, uint32_t e,
+                                          uint32_t last, uint32_t q, double lambda0, double mu0,
+                                          ElasRec& r) {
+    const uint32_t ec = min(e, last), en = min(e + 3, last);
+    rec_load<EC>(G, aent[ec], slots[(uint64_t)ec * 4 + q], q, lambda0, mu0, r);
+    r.nen = aent[en];
+    r.nk = slots[(uint64_t)en * 4 + q];
+}
+// add cell e's block into the accumulator (the nine old values are read
+// first, so the LDS round trip overlaps the arithmetic), then load the data
+// of cell e + 3 and the index of cell e + 6
+template <bool EC>
+__device__ __forceinline__ void rec_step(const double* __restrict__ G,
+                                         const uint32_t* __restrict__ aent,
+                                         const uint8_t* __restrict__ slots, uint32_t e,
+                                         uint32_t last, uint32_t q, double lambda0, double mu0,
+                                         uint32_t k0, uint32_t kmax, double* acc, uint32_t racc,
+                                         ElasRec& r) {
+    if (!(r.vol < 0.0) && r.k >= k0 && r.k < kmax) {
+        // components j of one block are 64 doubles apart
+        double* ab = acc + elas_acc_index(racc, r.k - k0, 0);
+        double old[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) old[j] = ab[j * 64];
+#pragma unroll
+        for (int al = 0; al < 3; ++al)
+#pragma unroll
+            for (int be = 0; be < 3; ++be)
+                ab[(al * 3 + be) * 64] =
+                    old[al * 3 + be] + elas_entry(r.ga, r.gb, al, be, r.lam, r.mu, r.vol);
+    }
+    const uint32_t en = min(e + 6, last);
+    rec_load<EC>(G, r.nen, r.nk, q, lambda0, mu0, r);
+    r.nen = aent[en];
+    r.nk = slots[(uint64_t)en * 4 + q];
+}
+
+template <bool EC>
+__global__ __launch_bounds__(64) void k_asm_elasticity_rows(
+    const double* __restrict__ G, const uint32_t* __restrict__ aptr,
+    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
+    double lambda0, double mu0, uint32_t wp, uint32_t nunits, uint32_t per,
+    double* __restrict__ vals) {
+    extern __shared__ double acc[];
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const uint32_t u = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (u >= nunits) return;
+    const uint32_t sl = u / 4, h = u % 4, lane = threadIdx.x;
+    const uint32_t q = lane & 3, row = sl * SLICE + h * ELAS_RPW + (lane >> 2);
+    const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
+    const bool live = row < nv;
+    const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
+    // the row and value pair this lane writes out
+    const uint32_t wr = lane % ELAS_RPW, wpp = lane / ELAS_RPW;
+    const uint32_t wrow = sl * SLICE + h * ELAS_RPW + wr;
+    const uint32_t wlen = wrow < nv ? rlen[wrow] : 0;
+    const uint32_t racc = lane >> 2;
+    for (uint32_t k0 = 0; k0 < w; k0 += wp) {
+        const uint32_t nk = min(wp, w - k0);
+        for (uint32_t i = lane; i < elas_acc_size(nk); i += SLICE) acc[i] = -0.0;
+        __syncthreads();
+        if (e0 < e1) {
+            const uint32_t last = e1 - 1;
+            // three records in rotation: record i holds the data of the cell
+            // it processes next and the index of the cell three after that,
+            // so data arrive three steps and indices six steps ahead of use
+            ElasRec r0, r1, r2;
+            rec_start<EC>(G, aent, slots, e0, last, q, lambda0, mu0, r0);
+            rec_start<EC>(G, aent, slots, e0 + 1, last, q, lambda0, mu0, r1);
+            rec_start<EC>(G, aent, slots, e0 + 2, last, q, lambda0, mu0, r2);
+            const uint32_t kmax = nk + k0;
+            uint32_t e = e0;
+            for (; e + 2 < e1; e += 3) {
+                rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
+                rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
+                rec_step<EC>(G, aent, slots, e + 2, last, q, lambda0, mu0, k0, kmax, acc, racc, r2);
+            }
+            if (e < e1)
+                rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
+            if (e + 1 < e1)
+                rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
+        }
+        __syncthreads();
+        // this pass's blocks: lane (wpp, wr) stores the component pair wpp of
+        // row wr, lanes 0-15 also component 8; zeros past the row's length
+        for (uint32_t kk = 0; kk < nk; ++kk) {
+            const uint32_t k = k0 + kk;
+            double* vk = vals + ((uint64_t)base + (uint64_t)k * SLICE) * 9;
+            const bool on = k < wlen;
+            const double* ak = acc + elas_acc_index(wr, kk, 0);
+            d2 v;
+            v.x = on ? ak[(2 * wpp) * 64] : 0.0;
+            v.y = on ? ak[(2 * wpp + 1) * 64] : 0.0;
+            *reinterpret_cast<d2*>(vk + 128 * wpp + 2 * (h * ELAS_RPW + wr)) = v;
+            if (wpp == 0) vk[512 + h * ELAS_RPW + wr] = on ? ak[8 * 64] : 0.0;
+        }
+        __syncthreads();
     }
 }
 
@@ -596,10 +768,36 @@ void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ece
                               fh_matrix* a, hipStream_t s) {
     Adjacency& adj = mesh_adjacency(m);
     const Pattern& p = *a->pat;
-    hipLaunchKernelGGL(k_asm_elasticity, dim3(cdiv(m->nv, 256)), dim3(256), 0, s, m->verts.p,
-                       m->cells.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
-                       m->nv, E, nu, Ecell_dev, a->vals.p);
+    // per-cell records of 16 doubles (one 128-byte line): g (12), vol, and
+    // lambda, mu with a per-cell E
+    const bool ec = Ecell_dev != nullptr;
+    DBuf<double> geom(a->dev, (size_t)m->nc * ELAS_GS);
+    if (m->nc) {
+        if (ec)
+            hipLaunchKernelGGL((k_elas_cell_geom<true>), dim3(cdiv(m->nc, 256)), dim3(256), 0,
+                               s, m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, geom.p);
+        else
+            hipLaunchKernelGGL((k_elas_cell_geom<false>), dim3(cdiv(m->nc, 256)), dim3(256), 0,
+                               s, m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, geom.p);
+        FH_LAUNCH_CHECK("elas_cell_geom");
+    }
+    // blocks per row held in LDS at once (48 blocks: 58.8 KB); wider rows take passes
+    const uint32_t wp = std::max(1u, std::min(p.max_width, 48u));
+    const size_t lds = (size_t)elas_acc_size(wp) * sizeof(double);
+    const uint32_t nunits = p.nslices * (SLICE / ELAS_RPW);
+    const uint32_t per = cdiv(nunits, 8u);
+    if (!nunits) return;
+    if (ec)
+        hipLaunchKernelGGL((k_asm_elasticity_rows<true>), dim3(8 * per), dim3(SLICE), lds, s,
+                           geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
+                           m->nv, 0.0, 0.0, wp, nunits, per, a->vals.p);
+    else
+        hipLaunchKernelGGL((k_asm_elasticity_rows<false>), dim3(8 * per), dim3(SLICE), lds, s,
+                           geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
+                           m->nv, lame_lambda(E, nu), lame_mu(E, nu), wp, nunits, per, a->vals.p);
     FH_LAUNCH_CHECK("asm_elasticity");
+    // geom goes back to the allocator's cache; the next user of that memory is
+    // ordered after these kernels on the same stream
 }
 
 fh_matrix* assemble_elasticity(fh_mesh* m, double E, double nu, const double* Ecell_dev) {
