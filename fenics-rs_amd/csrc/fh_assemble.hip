@@ -138,8 +138,7 @@ __global__ __launch_bounds__(256) void k_asm_p1_tet(
 //    in ascending order, so every value is the oracle's sum, bit for bit.
 //    The accumulator holds `wp` blocks per row; a slice wider than that is
 //    done in passes over ranges of wp blocks.  Work unit u = 4 * slice +
-//    quarter; block b of the grid takes unit (b % 8) * per + b / 8, so the
-//    units one XCD runs are consecutive and share cells in that XCD's L2.
+//    quarter; each block loops over units of its XCD (row_unit).
 constexpr int ELAS_GS = 16;  // doubles per cell record
 
 template <bool EC>
@@ -191,6 +190,20 @@ __global__ __launch_bounds__(256) void k_elas_cell_geom(const double* __restrict
     const uint32_t npieces = min(256u, nc - c0) * NP;
     d2* out = reinterpret_cast<d2*>(G + (uint64_t)c0 * ELAS_GS);
     for (uint32_t i = threadIdx.x; i < npieces; i += 256) out[i] = stage[(i / NP) * SP + i % NP];
+}
+
+// Work units of the row kernels (16 rows of a slice) are dealt to the 8 XCDs
+// in chunks of ROW_CHUNK consecutive units, round robin, so rows that share
+// cells run on one XCD (its L2) and a range of heavy rows (the vertex dofs of
+// a P2 numbering) spreads over all XCDs.  XCD x's j-th unit:
+constexpr uint32_t ROW_CHUNK = 16;
+__device__ __forceinline__ uint32_t row_unit(uint32_t x, uint32_t j) {
+    return ((j / ROW_CHUNK) * 8 + x) * ROW_CHUNK + j % ROW_CHUNK;
+}
+// units each XCD takes (its last chunk may run past the end: units >= nunits
+// are not processed)
+inline uint32_t row_units_per_xcd(uint32_t nunits) {
+    return (uint32_t)cdiv(nunits, 8 * ROW_CHUNK) * ROW_CHUNK;
 }
 
 constexpr int ELAS_RPW = 16;  // rows per wave of k_asm_elasticity_rows
@@ -274,58 +287,61 @@ __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
     double* __restrict__ vals) {
     extern __shared__ double acc[];
     typedef double d2 __attribute__((ext_vector_type(2)));
-    const uint32_t u = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (u >= nunits) return;
-    const uint32_t sl = u / 4, h = u % 4, lane = threadIdx.x;
-    const uint32_t q = lane & 3, row = sl * SLICE + h * ELAS_RPW + (lane >> 2);
-    const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
-    const bool live = row < nv;
-    const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
-    // the row and value pair this lane writes out
-    const uint32_t wr = lane % ELAS_RPW, wpp = lane / ELAS_RPW;
-    const uint32_t wrow = sl * SLICE + h * ELAS_RPW + wr;
-    const uint32_t wlen = wrow < nv ? rlen[wrow] : 0;
-    const uint32_t racc = lane >> 2;
-    for (uint32_t k0 = 0; k0 < w; k0 += wp) {
-        const uint32_t nk = min(wp, w - k0);
-        for (uint32_t i = lane; i < elas_acc_size(nk); i += SLICE) acc[i] = -0.0;
-        __syncthreads();
-        if (e0 < e1) {
-            const uint32_t last = e1 - 1;
-            // three records in rotation: record i holds the data of the cell
-            // it processes next and the index of the cell three after that,
-            // so data arrive three steps and indices six steps ahead of use
-            ElasRec r0, r1, r2;
-            rec_start<EC>(G, aent, slots, e0, last, q, lambda0, mu0, r0);
-            rec_start<EC>(G, aent, slots, e0 + 1, last, q, lambda0, mu0, r1);
-            rec_start<EC>(G, aent, slots, e0 + 2, last, q, lambda0, mu0, r2);
-            const uint32_t kmax = nk + k0;
-            uint32_t e = e0;
-            for (; e + 2 < e1; e += 3) {
-                rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
-                rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
-                rec_step<EC>(G, aent, slots, e + 2, last, q, lambda0, mu0, k0, kmax, acc, racc, r2);
+    // this XCD's units, strided by the XCD's blocks
+    for (uint32_t j = blockIdx.x / 8; j < per; j += gridDim.x / 8) {
+        const uint32_t u = row_unit(blockIdx.x % 8, j);
+        if (u >= nunits) break;
+        const uint32_t sl = u / 4, h = u % 4, lane = threadIdx.x;
+        const uint32_t q = lane & 3, row = sl * SLICE + h * ELAS_RPW + (lane >> 2);
+        const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
+        const bool live = row < nv;
+        const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
+        // the row and value pair this lane writes out
+        const uint32_t wr = lane % ELAS_RPW, wpp = lane / ELAS_RPW;
+        const uint32_t wrow = sl * SLICE + h * ELAS_RPW + wr;
+        const uint32_t wlen = wrow < nv ? rlen[wrow] : 0;
+        const uint32_t racc = lane >> 2;
+        for (uint32_t k0 = 0; k0 < w; k0 += wp) {
+            const uint32_t nk = min(wp, w - k0);
+            for (uint32_t i = lane; i < elas_acc_size(nk); i += SLICE) acc[i] = -0.0;
+            __syncthreads();
+            if (e0 < e1) {
+                const uint32_t last = e1 - 1;
+                // three records in rotation: record i holds the data of the cell
+                // it processes next and the index of the cell three after that,
+                // so data arrive three steps and indices six steps ahead of use
+                ElasRec r0, r1, r2;
+                rec_start<EC>(G, aent, slots, e0, last, q, lambda0, mu0, r0);
+                rec_start<EC>(G, aent, slots, e0 + 1, last, q, lambda0, mu0, r1);
+                rec_start<EC>(G, aent, slots, e0 + 2, last, q, lambda0, mu0, r2);
+                const uint32_t kmax = nk + k0;
+                uint32_t e = e0;
+                for (; e + 2 < e1; e += 3) {
+                    rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
+                    rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
+                    rec_step<EC>(G, aent, slots, e + 2, last, q, lambda0, mu0, k0, kmax, acc, racc, r2);
+                }
+                if (e < e1)
+                    rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
+                if (e + 1 < e1)
+                    rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
             }
-            if (e < e1)
-                rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
-            if (e + 1 < e1)
-                rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
+            __syncthreads();
+            // this pass's blocks: lane (wpp, wr) stores the component pair wpp of
+            // row wr, lanes 0-15 also component 8; zeros past the row's length
+            for (uint32_t kk = 0; kk < nk; ++kk) {
+                const uint32_t k = k0 + kk;
+                double* vk = vals + ((uint64_t)base + (uint64_t)k * SLICE) * 9;
+                const bool on = k < wlen;
+                const double* ak = acc + elas_acc_index(wr, kk, 0);
+                d2 v;
+                v.x = on ? ak[(2 * wpp) * 64] : 0.0;
+                v.y = on ? ak[(2 * wpp + 1) * 64] : 0.0;
+                *reinterpret_cast<d2*>(vk + 128 * wpp + 2 * (h * ELAS_RPW + wr)) = v;
+                if (wpp == 0) vk[512 + h * ELAS_RPW + wr] = on ? ak[8 * 64] : 0.0;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        // this pass's blocks: lane (wpp, wr) stores the component pair wpp of
-        // row wr, lanes 0-15 also component 8; zeros past the row's length
-        for (uint32_t kk = 0; kk < nk; ++kk) {
-            const uint32_t k = k0 + kk;
-            double* vk = vals + ((uint64_t)base + (uint64_t)k * SLICE) * 9;
-            const bool on = k < wlen;
-            const double* ak = acc + elas_acc_index(wr, kk, 0);
-            d2 v;
-            v.x = on ? ak[(2 * wpp) * 64] : 0.0;
-            v.y = on ? ak[(2 * wpp + 1) * 64] : 0.0;
-            *reinterpret_cast<d2*>(vk + 128 * wpp + 2 * (h * ELAS_RPW + wr)) = v;
-            if (wpp == 0) vk[512 + h * ELAS_RPW + wr] = on ? ak[8 * 64] : 0.0;
-        }
-        __syncthreads();
     }
 }
 
@@ -453,43 +469,194 @@ __global__ void k_fold_count(const uint32_t* __restrict__ count, uint32_t n, uin
 // P2 tetrahedron stiffness, FP64 VALU: K(a,b) = sum_q (w_q |det J|) (g_a^q . g_b^q)
 // with g^q = J^{-T} grad phi(xi_q) (lagrange.rs:324-346 at the
 // quadrature.rs:157-184 points).  G = reference gradients [q][dof][3].
+//
+// k_p2_elem_f64 computes each cell's 10 x 10 matrix once (one thread per
+// cell; the 55 entries a <= b, since g_a . g_b and g_b . g_a are the same
+// products summed in the same order) into Ke [nc][10][10]; k_gather_p2_rows
+// then sums the rows into the SELL values.  A cell the reference skips
+// (|det J| < 1e-14) has every entry set to P2_SKIP_BITS, a signalling NaN
+// that no arithmetic produces.
 struct P2Tab {
     double G[4][10][3];
     double w[4];
 };
-__global__ __launch_bounds__(256) void k_asm_p2_valu(
-    const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
-    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nd, P2Tab tab,
-    double* __restrict__ vals) {
-    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= nd) return;
-    const RowCtx rc = row_ctx(soff, rlen, d);
-    init_row<1>(vals, rc);
-    for (uint32_t e = aptr[d]; e < aptr[d + 1]; ++e) {
-        const uint32_t c = aent[e] / 10, a = aent[e] % 10;
+constexpr uint64_t P2_SKIP_BITS = 0x7ff0000000000badull;
+
+__global__ __launch_bounds__(64) void k_p2_elem_f64(const double* __restrict__ V,
+                                                    const uint32_t* __restrict__ C, uint32_t nc,
+                                                    P2Tab tab, double* __restrict__ Ke) {
+    __shared__ double stage[16 * 100];
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const uint32_t lane = threadIdx.x, c0 = blockIdx.x * 64, c = c0 + lane;
+    double K[55];
+#pragma unroll
+    for (int i = 0; i < 55; ++i) K[i] = __longlong_as_double((long long)P2_SKIP_BITS);
+    Map3 m;
+    bool ok = false;
+    if (c < nc) {
         double x[4][3];
         uint32_t t[4];
         load_cell<3, 4>(V, C, c, x, t);
-        Map3 m;
-        if (!affine3(x[0], x[1], x[2], x[3], m)) continue;
+        ok = affine3(x[0], x[1], x[2], x[3], m);
+    }
+    if (ok) {
         const double detj = fabs(m.det);
-        double ga[4][3], s[4];
+#pragma unroll
+        for (int i = 0; i < 55; ++i) K[i] = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            s[q] = tab.w[q] * detj;
-            grad3(m, tab.G[q][a], ga[q]);
-        }
-        for (int b = 0; b < 10; ++b) {
-            double acc = 0.0;
+            const double sq = tab.w[q] * detj;
+            double g[10][3];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double gb[3];
-                grad3(m, tab.G[q][b], gb);
-                acc += s[q] * (ga[q][0] * gb[0] + ga[q][1] * gb[1] + ga[q][2] * gb[2]);
-            }
-            vals[vidx(rc.soff, slots[(uint64_t)e * 10 + b], 0, rc.lane, 1)] += acc;
+            for (int b = 0; b < 10; ++b) grad3(m, tab.G[q][b], g[b]);
+            int i = 0;
+#pragma unroll
+            for (int a = 0; a < 10; ++a)
+#pragma unroll
+                for (int b = a; b < 10; ++b, ++i)
+                    K[i] += sq * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
         }
+    }
+    // four rounds of 16 cells: the round's lanes expand their matrices into
+    // LDS, then the wave stores the 16 matrices as one contiguous range
+#pragma unroll 1
+    for (uint32_t rr = 0; rr < 4; ++rr) {
+        if (lane / 16 == rr) {
+            double* o = stage + (lane % 16) * 100;
+            int i = 0;
+#pragma unroll
+            for (int a = 0; a < 10; ++a)
+#pragma unroll
+                for (int b = a; b < 10; ++b, ++i) {
+                    o[a * 10 + b] = K[i];
+                    o[b * 10 + a] = K[i];
+                }
+        }
+        __syncthreads();
+        const uint32_t cr = c0 + rr * 16;
+        if (cr < nc) {
+            const uint32_t np = min(16u, nc - cr) * 50;
+            d2* out = reinterpret_cast<d2*>(Ke + (uint64_t)cr * 100);
+            const d2* st = reinterpret_cast<const d2*>(stage);
+            for (uint32_t i = lane; i < np; i += 64) out[i] = st[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Row gather of element matrices Ke [nc][10][10] (T = double: the FP64
+// matrices, skipped cells marked; T = float: the MFMA matrices) into the
+// SELL-64 values of T.  One wave takes 16 rows of a slice, four lanes per row:
+// lane q adds entries (2q, 2q+1) of the row's element-matrix row, lanes 0 and
+// 1 also entry 8 + q, so a row's 10 entries arrive as four 16-byte (8-byte for
+// float) loads and two small ones.  Rows are summed in LDS (p2_acc_index,
+// banks as in the elasticity kernel) in ascending cell order per slot, and
+// written once.  A slot map of u8 bounds a row at 255 entries, so one pass
+// always covers the row.
+__device__ __forceinline__ uint32_t p2_acc_index(uint32_t r, uint32_t k) {
+    return ((k / 4) * 2 + r / 8) * 32 + (r % 8) * 4 + k % 4;
+}
+__device__ __forceinline__ bool p2_skipped(double v) {
+    return (uint64_t)__double_as_longlong(v) == P2_SKIP_BITS;
+}
+__device__ __forceinline__ bool p2_skipped(float) { return false; }
+
+// lane q's part of one incident cell's element-matrix row: entries (2q,
+// 2q+1) and, for q < 2, 8 + q, their slots, and the index entry of the cell
+// this record takes after it
+template <typename T>
+struct P2Rec {
+    T v0, v1, v2;
+    uint32_t s01, s2, nen, ne;
+};
+template <typename T>
+__device__ __forceinline__ void p2_load(const T* __restrict__ Ke, const uint8_t* __restrict__ slots,
+                                        uint32_t en, uint32_t e, uint32_t q, P2Rec<T>& r) {
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    const T* kr = Ke + (uint64_t)(en / 10) * 100 + (en % 10) * 10;
+    const t2 v = *reinterpret_cast<const t2*>(kr + 2 * q);
+    r.v0 = v.x;
+    r.v1 = v.y;
+    r.s01 = *reinterpret_cast<const uint16_t*>(slots + (uint64_t)e * 10 + 2 * q);
+    if (q < 2) {
+        r.v2 = kr[8 + q];
+        r.s2 = slots[(uint64_t)e * 10 + 8 + q];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void p2_start(const T* __restrict__ Ke, const uint32_t* __restrict__ aent,
+                                         const uint8_t* __restrict__ slots, uint32_t e,
+                                         uint32_t last, uint32_t q, P2Rec<T>& r) {
+    const uint32_t ec = min(e, last);
+    p2_load(Ke, slots, aent[ec], ec, q, r);
+    r.ne = min(e + 3, last);
+    r.nen = aent[r.ne];
+}
+template <typename T>
+__device__ __forceinline__ void p2_step(const T* __restrict__ Ke, const uint32_t* __restrict__ aent,
+                                        const uint8_t* __restrict__ slots, uint32_t e,
+                                        uint32_t last, uint32_t q, uint32_t r, T* acc, P2Rec<T>& x) {
+    if (!p2_skipped(x.v0)) {
+        T* a0 = acc + p2_acc_index(r, x.s01 & 255u);
+        T* a1 = acc + p2_acc_index(r, x.s01 >> 8);
+        const T o0 = *a0, o1 = *a1;
+        *a0 = o0 + x.v0;
+        *a1 = o1 + x.v1;
+        if (q < 2) {
+            T* a2 = acc + p2_acc_index(r, x.s2);
+            *a2 = *a2 + x.v2;
+        }
+    }
+    p2_load(Ke, slots, x.nen, x.ne, q, x);
+    x.ne = min(e + 6, last);
+    x.nen = aent[x.ne];
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_gather_p2_rows(
+    const T* __restrict__ Ke, const uint32_t* __restrict__ aptr, const uint32_t* __restrict__ aent,
+    const uint8_t* __restrict__ slots, const uint32_t* __restrict__ soff,
+    const uint32_t* __restrict__ rlen, uint32_t nd, uint32_t nunits, uint32_t per,
+    T* __restrict__ vals) {
+    extern __shared__ __align__(16) unsigned char smem_p2[];
+    T* acc = reinterpret_cast<T*>(smem_p2);
+    // this XCD's units, strided by the XCD's blocks
+    for (uint32_t j = blockIdx.x / 8; j < per; j += gridDim.x / 8) {
+        const uint32_t u = row_unit(blockIdx.x % 8, j);
+        if (u >= nunits) break;
+        const uint32_t sl = u / 4, h = u % 4, lane = threadIdx.x;
+        const uint32_t q = lane & 3, r = lane >> 2, row = sl * SLICE + h * 16 + r;
+        const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
+        const bool live = row < nd;
+        const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
+        const uint32_t accn = ((w + 3) / 4) * 64;
+        for (uint32_t i = lane; i < accn; i += SLICE) acc[i] = (T)-0.0;
+        __syncthreads();
+        if (e0 < e1) {
+            // three records in rotation, as in k_asm_elasticity_rows: data three
+            // cells and indices six cells ahead of use
+            const uint32_t last = e1 - 1;
+            P2Rec<T> r0, r1, r2;
+            p2_start(Ke, aent, slots, e0, last, q, r0);
+            p2_start(Ke, aent, slots, e0 + 1, last, q, r1);
+            p2_start(Ke, aent, slots, e0 + 2, last, q, r2);
+            uint32_t e = e0;
+            for (; e + 2 < e1; e += 3) {
+                p2_step(Ke, aent, slots, e, last, q, r, acc, r0);
+                p2_step(Ke, aent, slots, e + 1, last, q, r, acc, r1);
+                p2_step(Ke, aent, slots, e + 2, last, q, r, acc, r2);
+            }
+            if (e < e1) p2_step(Ke, aent, slots, e, last, q, r, acc, r0);
+            if (e + 1 < e1) p2_step(Ke, aent, slots, e + 1, last, q, r, acc, r1);
+        }
+        __syncthreads();
+        // lane (kq, wr) stores blocks kq, kq + 4, ... of row wr; zeros past its length
+        const uint32_t wr = lane % 16, wrow = sl * SLICE + h * 16 + wr;
+        const uint32_t wlen = wrow < nd ? rlen[wrow] : 0;
+        for (uint32_t k = lane / 16; k < w; k += 4)
+            vals[(uint64_t)base + (uint64_t)k * SLICE + h * 16 + wr] =
+                k < wlen ? acc[p2_acc_index(wr, k)] : (T)0;
+        __syncthreads();  // the next unit clears the accumulator
     }
 }
 
@@ -719,6 +886,37 @@ static P2Tab p2_table() {
     return t;
 }
 
+// Blocks of the row kernels (one wave each, looping over the units of
+// their XCD): FH_ASM_WAVES_PER_CU (default 8) per compute unit, at most one
+// per unit.  A multiple of 8, so block b stays on XCD b % 8.
+static uint32_t rows_grid(int dev, uint32_t per) {
+    static int wpc = [] {
+        const char* e = std::getenv("FH_ASM_WAVES_PER_CU");
+        return e ? std::max(1, std::atoi(e)) : 8;
+    }();
+    const uint32_t cus = (uint32_t)std::max(8, device(dev).prop.multiProcessorCount);
+    return 8 * std::max(1u, std::min(per, cus / 8 * (uint32_t)wpc));
+}
+
+void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, hipStream_t s) {
+    MeshP2& p2 = mesh_p2(m);
+    const Pattern& pat = *p2.pat;
+    const uint32_t nunits = pat.nslices * 4, per = row_units_per_xcd(nunits);
+    if (!nunits) return;
+    const size_t accn = (size_t)((pat.max_width + 3) / 4) * 64;
+    if (f64)
+        hipLaunchKernelGGL(k_gather_p2_rows<double>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
+                           accn * sizeof(double), s, (const double*)Ke, p2.adj->ptr.p,
+                           p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
+                           p2.adj->ndofs, nunits, per, (double*)vals);
+    else
+        hipLaunchKernelGGL(k_gather_p2_rows<float>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
+                           accn * sizeof(float), s, (const float*)Ke, p2.adj->ptr.p,
+                           p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
+                           p2.adj->ndofs, nunits, per, (float*)vals);
+    FH_LAUNCH_CHECK("gather_p2_rows");
+}
+
 // P2 basis values at the degree-2 points, [q][10] (lagrange.rs:301-320)
 static std::vector<double> p2_basis_table() {
     const double s5 = std::sqrt(5.0);
@@ -785,14 +983,14 @@ void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ece
     const uint32_t wp = std::max(1u, std::min(p.max_width, 48u));
     const size_t lds = (size_t)elas_acc_size(wp) * sizeof(double);
     const uint32_t nunits = p.nslices * (SLICE / ELAS_RPW);
-    const uint32_t per = cdiv(nunits, 8u);
+    const uint32_t per = row_units_per_xcd(nunits);
     if (!nunits) return;
     if (ec)
-        hipLaunchKernelGGL((k_asm_elasticity_rows<true>), dim3(8 * per), dim3(SLICE), lds, s,
+        hipLaunchKernelGGL((k_asm_elasticity_rows<true>), dim3(rows_grid(a->dev, per)), dim3(SLICE), lds, s,
                            geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
                            m->nv, 0.0, 0.0, wp, nunits, per, a->vals.p);
     else
-        hipLaunchKernelGGL((k_asm_elasticity_rows<false>), dim3(8 * per), dim3(SLICE), lds, s,
+        hipLaunchKernelGGL((k_asm_elasticity_rows<false>), dim3(rows_grid(a->dev, per)), dim3(SLICE), lds, s,
                            geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
                            m->nv, lame_lambda(E, nu), lame_mu(E, nu), wp, nunits, per, a->vals.p);
     FH_LAUNCH_CHECK("asm_elasticity");
@@ -1238,11 +1436,13 @@ int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out) {
         FH_CHECK(method >= 0 && method <= 2, "P2 method must be 0 (VALU FP64), 1 (MFMA FP32) "
                                              "or 2 (both: FP64 values + MFMA FP32 copy)");
         if (method != 1) {
-            hipLaunchKernelGGL(k_asm_p2_valu, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s,
-                               m->verts.p, m->cells.p, p2.adj->ptr.p, p2.adj->ent.p,
-                               p2.adj->slots.p, p2.pat->slice_off.p, p2.pat->row_len.p,
-                               p2.adj->ndofs, p2_table(), a->vals.p);
-            FH_LAUNCH_CHECK("asm_p2_valu");
+            DBuf<double> Ke(m->dev, (size_t)m->nc * 100);
+            if (m->nc) {
+                hipLaunchKernelGGL(k_p2_elem_f64, dim3(cdiv(m->nc, 64)), dim3(64), 0, s,
+                                   m->verts.p, m->cells.p, m->nc, p2_table(), Ke.p);
+                FH_LAUNCH_CHECK("p2_elem_f64");
+            }
+            p2_gather(m, Ke.p, true, a->vals.p, s);
         }
         if (method != 0) assemble_p2_mfma(m, a.get(), method == 1, s);
         FH_HIP(hipStreamSynchronize(s));

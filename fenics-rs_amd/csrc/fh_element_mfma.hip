@@ -14,7 +14,7 @@
 // v_mfma_f32_16x16x4f32: a wave owns 16 cells and sweeps 7 output tiles of 16
 // entries with two K=4 steps each.  C is computed per cell in FP32 (16 lanes),
 // staged through LDS; S lives in LDS for the block.  The element matrices go
-// to an [nc][100] FP32 buffer; k_gather_p2_f32 then sums them per dof row in
+// to an [nc][100] FP32 buffer; p2_gather then sums them per dof row in
 // ascending cell order (no atomics).
 #include <vector>
 
@@ -111,30 +111,6 @@ __global__ __launch_bounds__(256) void k_p2_elem_mfma(const double* __restrict__
     }
 }
 
-// Row-gather of FP32 element matrices into the SELL-64 FP32 values, ascending
-// cell order per slot (deterministic).
-__global__ __launch_bounds__(256) void k_gather_p2_f32(const uint32_t* __restrict__ aptr,
-                                                       const uint32_t* __restrict__ aent,
-                                                       const uint8_t* __restrict__ slots,
-                                                       const uint32_t* __restrict__ soff,
-                                                       const uint32_t* __restrict__ rlen, uint32_t nd,
-                                                       const float* __restrict__ Ke,
-                                                       float* __restrict__ vals) {
-    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= nd) return;
-    const uint32_t s = d / SLICE, lane = d % SLICE, base = soff[s], len = rlen[d];
-    for (uint32_t k = 0; k < len; ++k) vals[(uint64_t)base + (uint64_t)k * SLICE + lane] = -0.0f;
-    for (uint32_t e = aptr[d]; e < aptr[d + 1]; ++e) {
-        const uint32_t c = aent[e] / 10, a = aent[e] % 10;
-        const float* row = Ke + (uint64_t)c * 100 + a * 10;
-#pragma unroll
-        for (int b = 0; b < 10; ++b) {
-            const uint64_t ix = (uint64_t)base + (uint64_t)slots[(uint64_t)e * 10 + b] * SLICE + lane;
-            vals[ix] = vals[ix] + row[b];
-        }
-    }
-}
-
 __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -194,10 +170,7 @@ void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s) {
         a->vals32.alloc(m->dev, std::max<uint64_t>(nv, 1));
         fill_f32(a->vals32.p, a->vals32.n, 0.0f, s);  // zero SELL padding
     }
-    hipLaunchKernelGGL(k_gather_p2_f32, dim3(cdiv(p2.adj->ndofs, 256)), dim3(256), 0, s,
-                       p2.adj->ptr.p, p2.adj->ent.p, p2.adj->slots.p, p2.pat->slice_off.p,
-                       p2.pat->row_len.p, p2.adj->ndofs, Ke.p, a->vals32.p);
-    FH_LAUNCH_CHECK("gather_p2_f32");
+    p2_gather(m, Ke.p, false, a->vals32.p, s);
     if (fill64) {
         hipLaunchKernelGGL(k_f32_to_f64, dim3(std::max(1u, std::min<unsigned>(cdiv(nv, 256), 16384))),
                            dim3(256), 0, s, a->vals32.p, a->vals.p, nv);

@@ -138,6 +138,13 @@ def test_p2_dofmap_stiffness_rhs_bitwise():
     assert np.array_equal(bits(fh.assemble_p2_rhs_cellconst(gm, fc)), bits(p2.rhs_cellconst(fc)))
 
 
+def test_p2_stiffness_bitwise_many_slices():
+    """P2 stiffness over 77 SELL slices (unit_cube(8, 8, 8), 4913 dofs, vertex
+    and edge rows of different widths) is the oracle's, bit for bit."""
+    gm, om = fh.unit_cube(8, 8, 8), O.unit_cube(8, 8, 8)
+    same_csc(fh.assemble_p2_stiffness(gm, 0), O.P2(om).stiffness())
+
+
 # ---------------------------------------------------------------- full-size configurations
 COUNTS = json.load(open(os.path.join(GOLDEN, "counts.json")))
 
