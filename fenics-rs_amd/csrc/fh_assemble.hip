@@ -551,7 +551,8 @@ __global__ __launch_bounds__(64) void k_p2_elem_f64(const double* __restrict__ V
 // 1 also entry 8 + q, so a row's 10 entries arrive as four 16-byte (8-byte for
 // float) loads and two small ones.  Rows are summed in LDS (p2_acc_index,
 // banks as in the elasticity kernel) in ascending cell order per slot, and
-// written once.  A slot map of u8 bounds a row at 255 entries, so one pass
+// written once (every SELL entry, padding included; with vals64 also as
+// doubles).  A slot map of u8 bounds a row at 255 entries, so one pass
 // always covers the row.
 __device__ __forceinline__ uint32_t p2_acc_index(uint32_t r, uint32_t k) {
     return ((k / 4) * 2 + r / 8) * 32 + (r % 8) * 4 + k % 4;
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(64) void k_gather_p2_rows(
     const T* __restrict__ Ke, const uint32_t* __restrict__ aptr, const uint32_t* __restrict__ aent,
     const uint8_t* __restrict__ slots, const uint32_t* __restrict__ soff,
     const uint32_t* __restrict__ rlen, uint32_t nd, uint32_t nunits, uint32_t per,
-    T* __restrict__ vals) {
+    T* __restrict__ vals, double* __restrict__ vals64) {
     extern __shared__ __align__(16) unsigned char smem_p2[];
     T* acc = reinterpret_cast<T*>(smem_p2);
     // this XCD's units, strided by the XCD's blocks
@@ -653,9 +654,12 @@ __global__ __launch_bounds__(64) void k_gather_p2_rows(
         // lane (kq, wr) stores blocks kq, kq + 4, ... of row wr; zeros past its length
         const uint32_t wr = lane % 16, wrow = sl * SLICE + h * 16 + wr;
         const uint32_t wlen = wrow < nd ? rlen[wrow] : 0;
-        for (uint32_t k = lane / 16; k < w; k += 4)
-            vals[(uint64_t)base + (uint64_t)k * SLICE + h * 16 + wr] =
-                k < wlen ? acc[p2_acc_index(wr, k)] : (T)0;
+        for (uint32_t k = lane / 16; k < w; k += 4) {
+            const uint64_t ix = (uint64_t)base + (uint64_t)k * SLICE + h * 16 + wr;
+            const T v = k < wlen ? acc[p2_acc_index(wr, k)] : (T)0;
+            vals[ix] = v;
+            if (vals64) vals64[ix] = (double)v;
+        }
         __syncthreads();  // the next unit clears the accumulator
     }
 }
@@ -898,7 +902,7 @@ static uint32_t rows_grid(int dev, uint32_t per) {
     return 8 * std::max(1u, std::min(per, cus / 8 * (uint32_t)wpc));
 }
 
-void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, hipStream_t s) {
+void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, double* vals64, hipStream_t s) {
     MeshP2& p2 = mesh_p2(m);
     const Pattern& pat = *p2.pat;
     const uint32_t nunits = pat.nslices * 4, per = row_units_per_xcd(nunits);
@@ -908,12 +912,12 @@ void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, hipStream_t s) 
         hipLaunchKernelGGL(k_gather_p2_rows<double>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
                            accn * sizeof(double), s, (const double*)Ke, p2.adj->ptr.p,
                            p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
-                           p2.adj->ndofs, nunits, per, (double*)vals);
+                           p2.adj->ndofs, nunits, per, (double*)vals, nullptr);
     else
         hipLaunchKernelGGL(k_gather_p2_rows<float>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
                            accn * sizeof(float), s, (const float*)Ke, p2.adj->ptr.p,
                            p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
-                           p2.adj->ndofs, nunits, per, (float*)vals);
+                           p2.adj->ndofs, nunits, per, (float*)vals, vals64);
     FH_LAUNCH_CHECK("gather_p2_rows");
 }
 
@@ -1442,7 +1446,7 @@ int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out) {
                                    m->verts.p, m->cells.p, m->nc, p2_table(), Ke.p);
                 FH_LAUNCH_CHECK("p2_elem_f64");
             }
-            p2_gather(m, Ke.p, true, a->vals.p, s);
+            p2_gather(m, Ke.p, true, a->vals.p, nullptr, s);
         }
         if (method != 0) assemble_p2_mfma(m, a.get(), method == 1, s);
         FH_HIP(hipStreamSynchronize(s));

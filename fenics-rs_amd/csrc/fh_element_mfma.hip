@@ -111,12 +111,6 @@ __global__ __launch_bounds__(256) void k_p2_elem_mfma(const double* __restrict__
     }
 }
 
-__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        b[i] = (double)a[i];
-}
-
 // Reference tensors S_k [8][112] (rows 6,7 and columns >= 100 zero), FP64 on
 // the host from the lagrange.rs:324-346 gradients at the quadrature.rs:157-184
 // points, rounded to FP32.
@@ -165,17 +159,10 @@ void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s) {
     hipLaunchKernelGGL(k_p2_elem_mfma, dim3(g), dim3(256), 0, s, m->verts.p, m->cells.p, m->nc,
                        dS.p, Ke.p);
     FH_LAUNCH_CHECK("p2_elem_mfma");
-    const uint64_t nv = a->pat->sell_entries;
-    if (!a->vals32.p) {
-        a->vals32.alloc(m->dev, std::max<uint64_t>(nv, 1));
-        fill_f32(a->vals32.p, a->vals32.n, 0.0f, s);  // zero SELL padding
-    }
-    p2_gather(m, Ke.p, false, a->vals32.p, s);
-    if (fill64) {
-        hipLaunchKernelGGL(k_f32_to_f64, dim3(std::max(1u, std::min<unsigned>(cdiv(nv, 256), 16384))),
-                           dim3(256), 0, s, a->vals32.p, a->vals.p, nv);
-        FH_LAUNCH_CHECK("f32_to_f64");
-    }
+    // the gather writes every SELL entry (padding as zeros), in FP32 and, for
+    // fill64, in FP64
+    if (!a->vals32.p) a->vals32.alloc(m->dev, std::max<uint64_t>(a->pat->sell_entries, 1));
+    p2_gather(m, Ke.p, false, a->vals32.p, fill64 ? a->vals.p : nullptr, s);
     FH_HIP(hipStreamSynchronize(s));
 }
 

@@ -153,8 +153,9 @@ struct HostCsr {
 };
 HostCsr export_scalar_csr(const fh_matrix* a);
 void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s);
-// sum P2 element matrices Ke [nc][10][10] (FP64, or FP32 when !f64) into SELL values
-void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, hipStream_t s);
+// sum P2 element matrices Ke [nc][10][10] (FP64, or FP32 when !f64) into SELL
+// values; FP32 sums are also written as doubles to vals64 when it is not null
+void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, double* vals64, hipStream_t s);
 uint64_t sell_value_count(const fh_matrix* a);  // entries * bs*bs
 
 // SpMV (fh_cg.hip)
