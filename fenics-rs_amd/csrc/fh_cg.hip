@@ -131,6 +131,7 @@ struct CgMat {
     const double* v64;
     const float* v32;
     const uint32_t* t64;  // (last value, column) records, see fh_matrix::tail64
+    const uint32_t* q32;  // FP32 3x3 blocks in 16-byte groups, see fh_matrix::quad32
     const double* S64;  // block row sums, lane-interleaved [slice][bs*bs][64]
     const float* S32;
     uint32_t nrows;
@@ -479,105 +480,9 @@ __device__ __forceinline__ void row_product_pipe(const CgMat& A, const TM* __res
 
 // The same for FP64 3x3 blocks, one block per step, with the block's last
 // value and column index in one 12-byte tail record loaded a step ahead: one
-// memory instruction fewer per block (measured -1.5% at C1; the FP32 and
-// scalar products measured 7-8% slower with tails, so they keep the form above).
-// FP32 3x3 product: the first step (column index and values) loaded before
-// the kernel's prologue, then the two-step pipeline from step 1 on.  Holding
-// one step across the prologue costs 10 VGPRs; two would cost 31.
-template <int BS, class TM>
-struct Pipe1Head {
-    uint32_t base, w, c;
-    TM v[BS * BS];
-};
-
-template <int BS, class TM, bool NT>
-__device__ __forceinline__ void pipe1_head(const CgMat& A, const TM* __restrict__ vals, uint32_t s,
-                                           uint32_t lane, Pipe1Head<BS, TM>& h) {
-    constexpr int B2 = BS * BS;
-    h.base = A.soff[s];
-    h.w = (A.soff[s + 1] - h.base) / SLICE;
-    if (h.w == 0) return;
-    h.c = ld_s<NT>(A.cols + (uint64_t)h.base + lane);
-    const TM* vk = vals + (uint64_t)h.base * B2;
-    if constexpr (B2 == 9) {
-#pragma unroll
-        for (int pp = 0; pp < 4; ++pp) ld_pair<NT>(vk + 128 * pp + 2 * lane, h.v[2 * pp], h.v[2 * pp + 1]);
-        h.v[8] = ld_s<NT>(vk + 512 + lane);
-    } else {
-#pragma unroll
-        for (int j = 0; j < B2; ++j) h.v[j] = ld_s<NT>(vk + j * SLICE + lane);
-    }
-}
-
-template <int BS, int U, class TM, bool DIFF, bool NT, class G>
-__device__ __forceinline__ void row_product_pipe_h1(const CgMat& A, const TM* __restrict__ vals,
-                                                    const Pipe1Head<BS, TM>& h, uint32_t lane,
-                                                    const double* xr, G&& gather, double* acc) {
-    constexpr int B2 = BS * BS;
-    const uint32_t base = h.base, w = h.w;
-#pragma unroll
-    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    if (w == 0) return;
-    const uint32_t* cp = A.cols + (uint64_t)base + lane;
-    const TM* vb = vals + (uint64_t)base * B2;  // step k: vb + k * SLICE * B2, see vpos
-    const uint32_t last = w - 1;
-    uint32_t c[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = ld_s<NT>(cp + (uint64_t)min(1u + u, last) * SLICE);
-    {
-        double xg[BS];
-        gather(h.c, xg);
-#pragma unroll
-        for (int q = 0; q < BS; ++q)
-            if (DIFF) xg[q] -= xr[q];
-#pragma unroll
-        for (int a = 0; a < BS; ++a)
-#pragma unroll
-            for (int q = 0; q < BS; ++q) acc[a] = fma((double)h.v[a * BS + q], xg[q], acc[a]);
-    }
-    for (uint32_t k = 1; k < w; k += U) {
-        uint32_t cn[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            cn[u] = ld_s<NT>(cp + (uint64_t)min(k + U + u, last) * SLICE);
-        TM v[U][B2];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const TM* vk = vb + (uint64_t)min(k + u, last) * SLICE * B2;
-            if constexpr (B2 == 9) {
-#pragma unroll
-                for (int pp = 0; pp < 4; ++pp)
-                    ld_pair<NT>(vk + 128 * pp + 2 * lane, v[u][2 * pp], v[u][2 * pp + 1]);
-                v[u][8] = ld_s<NT>(vk + 512 + lane);
-            } else {
-#pragma unroll
-                for (int j = 0; j < B2; ++j) v[u][j] = ld_s<NT>(vk + j * SLICE + lane);
-            }
-        }
-        double xg[U][BS];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            gather(c[u], xg[u]);
-            // steps past the slice width re-read its last entry: add zero
-            const bool ok = u == 0 || k + u < w;
-#pragma unroll
-            for (int q = 0; q < BS; ++q) {
-                const double d = DIFF ? xg[u][q] - xr[q] : xg[u][q];
-                xg[u][q] = ok ? d : 0.0;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int a = 0; a < BS; ++a)
-#pragma unroll
-                for (int q = 0; q < BS; ++q)
-                    acc[a] = fma((double)v[u][a * BS + q], xg[u][q], acc[a]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) c[u] = cn[u];
-    }
-}
-
+// memory instruction fewer per block (measured -1.5% at C1; the scalar
+// product measured 7-8% slower with tails and keeps the form above; the FP32
+// 3x3 product has a layout of its own, row_product_quad32).
 // The first step's loads (tail record and the eight paired values), which
 // depend on nothing the kernel computes: issued before the kernel's prologue.
 struct Tail3Head {
@@ -636,6 +541,78 @@ __device__ __forceinline__ void row_product_tail3(const CgMat& A, const Tail3Hea
         const double* vk = vb + (uint64_t)k * SLICE * 9;
 #pragma unroll
         for (int pp = 0; pp < 4; ++pp) ld_pair<true>(vk + 128 * pp + 2 * lane, v[2 * pp], v[2 * pp + 1]);
+        v[8] = vl;
+        step(v, c);
+        c = cn;
+        vl = vn;
+    }
+}
+
+// FP32 3x3 blocks in the grouped layout (fh_matrix::quad32): a block is two
+// 16-byte loads (values 0-7) and one 8-byte load (value 8 and the column),
+// 3 memory instructions instead of the 5 value loads and 1 column load of the
+// lane-pair layout.  The first step is loaded before the kernel's prologue,
+// the (value 8, column) record one step ahead.
+struct Quad32Head {
+    uint32_t base, w;  // slice start (entries) and width (steps)
+    uint32_t c;        // column of step 0
+    float v[9];        // values of step 0
+};
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void ld_quads(const uint32_t* qs, uint32_t lane, float* v) {
+    const f4v g0 = *reinterpret_cast<const f4v*>(qs + 4 * lane);
+    const f4v g1 = *reinterpret_cast<const f4v*>(qs + 256 + 4 * lane);
+    v[0] = g0.x; v[1] = g0.y; v[2] = g0.z; v[3] = g0.w;
+    v[4] = g1.x; v[5] = g1.y; v[6] = g1.z; v[7] = g1.w;
+}
+__device__ __forceinline__ void ld_q32_tail(const uint32_t* qs, uint32_t lane, float& v8,
+                                            uint32_t& c) {
+    const u2v t = *reinterpret_cast<const u2v*>(qs + 512 + 2 * lane);
+    v8 = __uint_as_float(t.x);
+    c = t.y;
+}
+
+__device__ __forceinline__ void quad32_head(const CgMat& A, uint32_t s, uint32_t lane,
+                                            Quad32Head& h) {
+    h.base = A.soff[s];
+    h.w = (A.soff[s + 1] - h.base) / SLICE;
+    if (h.w == 0) return;
+    const uint32_t* qs = A.q32 + (uint64_t)h.base * 10;
+    ld_q32_tail(qs, lane, h.v[8], h.c);
+    ld_quads(qs, lane, h.v);
+}
+
+template <class G>
+__device__ __forceinline__ void row_product_quad32(const CgMat& A, const Quad32Head& h,
+                                                   uint32_t lane, const double* xr, G&& gather,
+                                                   double* acc) {
+    const uint32_t w = h.w;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = 0.0;
+    if (w == 0) return;
+    const uint32_t* qb = A.q32 + (uint64_t)h.base * 10;  // step k: qb + k * 640
+    const uint32_t last = w - 1;
+    auto step = [&](const float* v, uint32_t c) {
+        double xg[3];
+        gather(c, xg);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xg[q] -= xr[q];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[a] = fma((double)v[a * 3 + q], xg[q], acc[a]);
+    };
+    float vl, vn;
+    uint32_t c, cn;
+    ld_q32_tail(qb + (uint64_t)min(1u, last) * 640, lane, vl, c);
+    step(h.v, h.c);
+    for (uint32_t k = 1; k < w; ++k) {
+        ld_q32_tail(qb + (uint64_t)min(k + 1, last) * 640, lane, vn, cn);
+        float v[9];
+        ld_quads(qb + (uint64_t)k * 640, lane, v);
         v[8] = vl;
         step(v, c);
         c = cn;
@@ -709,7 +686,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
     constexpr bool EARLY = EARLY64 || EARLY32;
     double xz[BS], po[BS];
     Tail3Head head;
-    Pipe1Head<BS, float> head32;
+    Quad32Head head32;
     auto own_pairs = [&] {
 #pragma unroll
         for (int q = 0; q < BS; ++q)
@@ -718,7 +695,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
     if (EARLY && active) {
         own_pairs();
         if constexpr (EARLY64) tail3_head(A, s, lane, head);
-        if constexpr (EARLY32) pipe1_head<BS, float, false>(A, A.v32, s, lane, head32);
+        if constexpr (EARLY32) quad32_head(A, s, lane, head32);
     }
     const uint32_t status = st->status, need_rep = st->need_rep;
     if (!EARLY && (status != ST_RUNNING || need_rep)) return;
@@ -881,7 +858,7 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
         constexpr int U = BS == 1 ? 4 : (LOW ? 2 : 1);
         if constexpr (LOW) {
             if constexpr (EARLY32)
-                row_product_pipe_h1<BS, U, float, true, false>(A, A.v32, head32, lane, xo, gather, acc);
+                row_product_quad32(A, head32, lane, xo, gather, acc);
             else
                 row_product_pipe<BS, U, float, true, false>(A, A.v32, s, lane, xo, gather, acc);
             add_rowsum<BS>(A.S32, s, lane, xo, acc);
@@ -1203,7 +1180,7 @@ __global__ void k_dd_add(double* __restrict__ hi, double* __restrict__ lo,
 static CgMat cgmat(const fh_matrix* a) {
     const Pattern& p = *a->pat;
     return {p.slice_off.p, p.row_len.p, p.sell_cols.p, a->vals.p, a->vals32.p, a->tail64.p,
-            a->rowsum64.p, a->rowsum32.p, p.nrows, p.nslices};
+            a->quad32.p, a->rowsum64.p, a->rowsum32.p, p.nrows, p.nslices};
 }
 
 static unsigned spmv_grid(const fh_matrix* a) { return std::max(1u, cdiv(a->pat->nslices, 4)); }
@@ -1276,6 +1253,35 @@ __global__ void k_make_tail(const double* __restrict__ vals, const uint32_t* __r
     tail[TAILW * e + 1] = (uint32_t)(b >> 32);
     tail[TAILW * e + 2] = cols[e];
     if constexpr (TAILW == 4) tail[TAILW * e + 3] = 0u;
+}
+
+// the FP32 3x3 blocks of every SELL entry e in the grouped layout of
+// fh_matrix::quad32, from the FP32 values (same rounding as the values)
+__global__ void k_make_quad32(const float* __restrict__ v32, const uint32_t* __restrict__ cols,
+                              uint64_t n, uint32_t* __restrict__ q) {
+    const uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t e0 = e & ~(uint64_t)(SLICE - 1);
+    const uint32_t lane = (uint32_t)(e & (SLICE - 1));
+    uint32_t* qs = q + e0 * 10;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        qs[(j < 4 ? 0 : 256) + 4 * lane + (j & 3)] = __float_as_uint(v32[vpos(e0, j, lane, 9)]);
+    qs[512 + 2 * lane] = __float_as_uint(v32[vpos(e0, 8, lane, 9)]);
+    qs[512 + 2 * lane + 1] = cols[e];
+}
+
+void ensure_quad32(fh_matrix* a) {
+    if (a->quad32.p || a->bs != 3) return;
+    ensure_vals32(a);
+    DeviceCtx& d = device(a->dev);
+    const uint64_t n = a->pat->sell_entries;
+    a->quad32.alloc(a->dev, std::max<uint64_t>(10 * n, 1));
+    if (n) {
+        hipLaunchKernelGGL(k_make_quad32, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0,
+                           d.stream, a->vals32.p, a->pat->sell_cols.p, n, a->quad32.p);
+        FH_LAUNCH_CHECK("make_quad32");
+    }
 }
 
 void ensure_tail(fh_matrix* a) {
@@ -1441,13 +1447,14 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     if (noalloc) {
         FH_CHECK(a->pat->xcd_ready && (!uses32 || a->vals32.p) &&
                      (!(uses32 || diff64) || a->rowsum64.p) && (!uses32 || a->rowsum32.p) &&
-                     (ORD || BS != 3 || a->tail64.p),
+                     (ORD || BS != 3 || a->tail64.p) && (!uses32 || BS != 3 || a->quad32.p),
                  "the allocation-free solve needs the matrix prepared for this precision: "
                  "solve it once with fh_solve_dev (or fh_solve) in the same precision first");
     } else {
         if (uses32) ensure_vals32(am);
         if (uses32 || diff64) ensure_rowsum(am, uses32);
         if (!ORD && BS == 3) ensure_tail(am);
+        if (uses32 && BS == 3) ensure_quad32(am);
     }
     XcdMap xm{};
     unsigned gc = 8;  // k_cg_spmv blocks

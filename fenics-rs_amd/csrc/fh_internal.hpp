@@ -88,6 +88,10 @@ struct fh_matrix {
     // 3x3 blocks: per SELL entry, the block's last value and its column in one
     // record (the FP64 CG product loads both with one instruction) [entries][3 words]
     fh::DBuf<uint32_t> tail64;
+    // 3x3 blocks, FP32 product of the CG: per SELL step of 64 entries, 640 words:
+    // values 0-3 of each lane as a 16-byte group (words 4 lane ..), values 4-7
+    // likewise (256 + 4 lane ..), then value 8 and the column (512 + 2 lane ..)
+    fh::DBuf<uint32_t> quad32;
     fh::DBuf<double> rowsum64;  // block row sums, lane-interleaved [slice][bs*bs][64] (difference form)
     fh::DBuf<float> rowsum32;   // FP32 copy of rowsum64
     fh::DBuf<uint8_t> fixed; // [n] Dirichlet mask after condensation (may be empty)
@@ -163,6 +167,7 @@ void spmv_f64(const fh_matrix* a, const double* x, double* y, int order, hipStre
 void inv_diag_f64(const fh_matrix* a, double* out, hipStream_t s);
 void ensure_rowsum(fh_matrix* a, bool f32);
 void ensure_tail(fh_matrix* a);
+void ensure_quad32(fh_matrix* a);
 
 // Solve (fh_cg.hip)
 struct SolveArgs {
