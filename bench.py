@@ -136,6 +136,25 @@ def p2_poisson(fh, np, dev):
     return out
 
 
+def profile_c2(fh, np, dev, iters):
+    """Counter runs of the C2 product: `iters` FP64_TRUE iterations of the C2
+    solve (not converged, by design), one dispatch per kernel launch."""
+    import torch
+    os.environ["FH_CG_GRAPH"] = "0"
+    gm = fh.unit_cube(64, 64, 64, device=dev)
+    fc = np.random.default_rng(64).random(gm.nc)
+    b = fh.assemble_p2_rhs_cellconst(gm, fc)
+    bd = gm.p2_boundary_dofs()
+    K = fh.assemble_p2_stiffness(gm, 0)
+    Kb, bb = fh.apply_dirichlet_bc(K, b, fh.DirichletBC(bd, np.zeros(len(bd))))
+    del K
+    bt = torch.tensor(bb, device=f"cuda:{dev}")
+    x = torch.empty_like(bt)
+    st = Kb.solve_dev(bt, x, tol=1e-30, max_iter=iters, precision=fh.FP64_TRUE, allow_fail=True)
+    torch.cuda.synchronize()
+    log(f"profile c2: {st.iterations} iterations")
+
+
 def batched_c4(fh, np):
     """C4: 256 unit_square(N_i, N_i) Poisson instances, N_i ~ U{32..512}
     (seed 2024), four nodal loads each (b = M f), tol 1e-10: block CG over the
@@ -215,6 +234,9 @@ def main():
                     help="profiling run: REPS assemblies of the cantilever elasticity stiffness "
                          "(uniform E, then per-cell E) and of the P2 Poisson unit_cube(64^3) "
                          "stiffness (FP64 VALU, then FP32 MFMA); prints the wall times, no JSON")
+    ap.add_argument("--profile-c2", type=int, default=0, metavar="ITERS",
+                    help="profiling run: ITERS FP64 CG iterations of the P2 Poisson "
+                         "unit_cube(64^3) solve; prints no JSON")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="profiling run: only PROFILE_ITERS CG iterations of the FP64-value "
                          "and of the FP32-value SpMV on the cantilever, then a 1 GiB device "
@@ -251,6 +273,9 @@ def main():
             f"{st.gflops:.2f} GDOF.iter/s")
         return
 
+    if args.profile_c2:
+        profile_c2(fh, np, dev, args.profile_c2)
+        return
     if args.assembly_once:
         assembly_once(fh, np, torch, dev, args.nx, args.assembly_once)
         return

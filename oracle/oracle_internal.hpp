@@ -59,6 +59,7 @@ struct or_p2 {
     std::vector<uint64_t> cells;     // corner connectivity [nc][4]
     std::vector<uint64_t> cell_dofs; // [nc][10]
     std::vector<std::array<uint64_t, 2>> edges;  // sorted (min, max)
+    std::vector<uint64_t> erank;  // edge dof number - nv, by position in edges
 };
 
 namespace orc {

@@ -4,8 +4,10 @@
 //   crates/fenics-solver/src/cg.rs:7-149   (spmv, dot, norm2, solve_cg)
 //   crates/fenics-solver/src/gpu.rs:1552-1583 (CSC -> CSR + inverse diagonal)
 // and the P2 tetrahedral dofmap / stiffness that the reference lacks:
-// vertices first, then edges in lexicographic (min, max) vertex order
-// (the vertices-then-edges convention of mass.rs:132-135); per-cell dofs in
+// vertices first, then edges (the vertices-then-edges convention of
+// mass.rs:132-135), the edges ordered by the number of cells around them,
+// most first (counts above 255 count as 255), ties in lexicographic
+// (min, max) vertex order; per-cell dofs in
 // P2Tetrahedron order (lagrange.rs:284-297: corners, then edges 01, 12, 02,
 // 03, 13, 23); K_e = sum_q (w_q |det J|) (g_i^q . g_j^q) with the degree-2
 // tetrahedron rule (quadrature.rs:157-184), exact for P2 x P2 gradients.
@@ -286,6 +288,12 @@ void or_csc_to_csr(const or_csc* A, uint32_t* row_ptr, uint32_t* col_idx, double
 
 static const int P2_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
 
+// position of edge {a, b} in the lexicographic list p->edges
+static uint64_t p2_edge_index(const or_p2* p, uint64_t a, uint64_t b) {
+    const std::array<uint64_t, 2> key = {std::min(a, b), std::max(a, b)};
+    return (uint64_t)(std::lower_bound(p->edges.begin(), p->edges.end(), key) - p->edges.begin());
+}
+
 or_p2* or_p2_build(uint64_t nv, const double* verts, uint64_t nc, const uint64_t* cells) {
     or_p2* p = new or_p2;
     p->nv = nv;
@@ -299,16 +307,33 @@ or_p2* or_p2_build(uint64_t nv, const double* verts, uint64_t nc, const uint64_t
             p->edges.push_back({std::min(a, b), std::max(a, b)});
         }
     std::sort(p->edges.begin(), p->edges.end());
-    p->edges.erase(std::unique(p->edges.begin(), p->edges.end()), p->edges.end());
+    // cells around each edge: the run length of its key in the sorted list
+    std::vector<uint64_t> around;
+    {
+        std::vector<std::array<uint64_t, 2>> uniq;
+        for (size_t i = 0; i < p->edges.size();) {
+            size_t j = i;
+            while (j < p->edges.size() && p->edges[j] == p->edges[i]) ++j;
+            uniq.push_back(p->edges[i]);
+            around.push_back(std::min<uint64_t>(j - i, 255));
+            i = j;
+        }
+        p->edges.swap(uniq);
+    }
     p->ne = p->edges.size();
+    // edge dof numbers: most cells around first, ties in lexicographic order
+    std::vector<uint64_t> order(p->ne);
+    for (uint64_t e = 0; e < p->ne; ++e) order[e] = e;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint64_t a, uint64_t b) { return around[a] > around[b]; });
+    p->erank.assign(p->ne, 0);
+    for (uint64_t r = 0; r < p->ne; ++r) p->erank[order[r]] = r;
     p->cell_dofs.resize(10 * nc);
     for (uint64_t c = 0; c < nc; ++c) {
         for (int a = 0; a < 4; ++a) p->cell_dofs[10 * c + a] = cells[4 * c + a];
         for (int k = 0; k < 6; ++k) {
             const uint64_t a = cells[4 * c + P2_EDGE[k][0]], b = cells[4 * c + P2_EDGE[k][1]];
-            const std::array<uint64_t, 2> key = {std::min(a, b), std::max(a, b)};
-            const auto it = std::lower_bound(p->edges.begin(), p->edges.end(), key);
-            p->cell_dofs[10 * c + 4 + k] = nv + (uint64_t)(it - p->edges.begin());
+            p->cell_dofs[10 * c + 4 + k] = nv + p->erank[p2_edge_index(p, a, b)];
         }
     }
     return p;
@@ -322,7 +347,7 @@ void or_p2_dof_coords(const or_p2* p, double* out) {
     std::copy(p->verts.begin(), p->verts.end(), out);
     for (uint64_t e = 0; e < p->ne; ++e)
         for (int d = 0; d < 3; ++d)
-            out[3 * (p->nv + e) + d] =
+            out[3 * (p->nv + p->erank[e]) + d] =
                 (p->verts[3 * p->edges[e][0] + d] + p->verts[3 * p->edges[e][1] + d]) * 0.5;
 }
 // Boundary P2 dofs: vertices and edges of the boundary facets (mesh.rs:145-210).
@@ -338,10 +363,7 @@ int or_p2_boundary_dofs(const or_p2* p, uint64_t* out, uint64_t* n_out) {
         for (int a = 0; a < 3; ++a) d.push_back(t[a]);
         const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
         for (auto& pr : pairs) {
-            const std::array<uint64_t, 2> key = {std::min(t[pr[0]], t[pr[1]]),
-                                                 std::max(t[pr[0]], t[pr[1]])};
-            const auto it = std::lower_bound(p->edges.begin(), p->edges.end(), key);
-            d.push_back(p->nv + (uint64_t)(it - p->edges.begin()));
+            d.push_back(p->nv + p->erank[p2_edge_index(p, t[pr[0]], t[pr[1]])]);
         }
     }
     std::sort(d.begin(), d.end());
