@@ -1364,6 +1364,13 @@ static void xcd_map(const Pattern& pc, hipStream_t s, XcdMap& xm, unsigned& gc) 
     gc = p.xcd_blocks;
 }
 
+CgCache::~CgCache() {
+    for (int g = 0; g < 2; ++g) {
+        if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+        if (graph[g]) (void)hipGraphDestroy(graph[g]);
+    }
+}
+
 // Device buffers of one solve, carved from one workspace.
 struct CgWork {
     double *r, *zp, *Ap, *xl, *bl, *xs0, *xs1, *dinv, *redA, *bb, *fl;
@@ -1462,10 +1469,30 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     const auto t_wall0 = std::chrono::steady_clock::now();
     const CgMat A = cgmat(a);
 
-    DBuf<uint8_t> own_ws;
+    const int M = std::max(1, env_int("FH_CG_CHUNK", 16));
+    const bool use_graph = !noalloc && env_int("FH_CG_GRAPH", 1) != 0;
+    // optional per-launch timing of the SpMV kernel: a start and a stop event
+    // carried by the kernel's own dispatch (hipExtLaunchKernelGGL), so the pair
+    // brackets its execution on this stream and nothing else
+    const bool time_spmv = !noalloc && env_int("FH_TIME_SPMV", 0) != 0;
+    // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
+    // the initial residual or a replacement, where the exact product restarts it
+    const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
+    // workspace and graphs: kept on the matrix per kernel configuration (the
+    // graphs reference only the matrix and the workspace), so a repeated solve
+    // of this operator allocates and captures nothing
+    CgCache* cc = nullptr;
     if (!noalloc) {
-        own_ws.alloc(dev, cg_work(a, PRE, DIFF, gc, nullptr, nullptr));
-        ws = own_ws.p;
+        const uint64_t key = (uint64_t)BS | (uint64_t)ORD << 4 | (uint64_t)DIFF << 5 |
+                             (uint64_t)PRE << 6 | (uint64_t)diff64 << 8 | (uint64_t)aprec << 9 |
+                             (uint64_t)M << 16 | (uint64_t)gc << 40;
+        std::unique_ptr<CgCache>& slot = am->cg_cache[key];
+        if (!slot) {
+            slot = std::make_unique<CgCache>();
+            slot->ws.alloc(dev, cg_work(a, PRE, DIFF, gc, nullptr, nullptr));
+        }
+        cc = slot.get();
+        ws = cc->ws.p;
     }
     CgWork w{};
     cg_work(a, PRE, DIFF, gc, ws, &w);
@@ -1508,12 +1535,6 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // the next spmv follows k_residual (gs partials) rather than an update (gu)
     bool after_res = true;
 
-    const int M = std::max(1, env_int("FH_CG_CHUNK", 16));
-    const bool use_graph = !noalloc && env_int("FH_CG_GRAPH", 1) != 0;
-    // optional per-launch timing of the SpMV kernel: a start and a stop event
-    // carried by the kernel's own dispatch (hipExtLaunchKernelGGL), so the pair
-    // brackets its execution on this stream and nothing else
-    const bool time_spmv = !noalloc && env_int("FH_TIME_SPMV", 0) != 0;
     const bool trace = env_int("FH_CG_TRACE", 0) != 0;  // per-chunk state on stderr
     std::vector<hipEvent_t> ev;
     double ms_hi = 0.0, ms_lo = 0.0;
@@ -1522,9 +1543,6 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         ev.resize(2 * M);
         for (auto& e : ev) FH_HIP(hipEventCreate(&e));
     }
-    // true-residual modes: Ap by recurrence (FH_CG_APREC=1) except right after
-    // the initial residual or a replacement, where the exact product restarts it
-    const bool aprec = DIFF && env_int("FH_CG_APREC", 0) != 0;
     auto launch_chunk = [&](bool res_first, bool lowk) {
         for (int m = 0; m < M; ++m) {
             const int nin = (m == 0 && res_first) ? (int)gs : (int)gu;
@@ -1562,21 +1580,19 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     uint64_t launched = 0;
     bool first_chunk = true;
     bool lowk = mode == M_MIXED_IR;  // operator precision of the next chunk
-    hipGraph_t graph2[2] = {nullptr, nullptr};
-    hipGraphExec_t gexec2[2] = {nullptr, nullptr};
     for (;;) {
         const uint64_t j0 = hs->it_upd;  // iterations done before this chunk
         // the first spmv after init / replacement reads gs residual partials;
         // later ones read gu update partials
         if (use_graph && !time_spmv && !first_chunk && !after_res) {
             const int g = lowk ? 1 : 0;
-            if (!gexec2[g]) {
+            if (!cc->exec[g]) {
                 FH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 launch_chunk(false, lowk);
-                FH_HIP(hipStreamEndCapture(s, &graph2[g]));
-                FH_HIP(hipGraphInstantiate(&gexec2[g], graph2[g], nullptr, nullptr, 0));
+                FH_HIP(hipStreamEndCapture(s, &cc->graph[g]));
+                FH_HIP(hipGraphInstantiate(&cc->exec[g], cc->graph[g], nullptr, nullptr, 0));
             }
-            FH_HIP(hipGraphLaunch(gexec2[g], s));
+            FH_HIP(hipGraphLaunch(cc->exec[g], s));
         } else {
             launch_chunk(after_res, lowk);
             FH_LAUNCH_CHECK("cg_chunk");
@@ -1622,10 +1638,6 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         if (launched > args.max_iter + (hs->n_rep + 2) * (uint64_t)M + 64) break;  // safety net
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int g = 0; g < 2; ++g) {
-        if (gexec2[g]) (void)hipGraphExecDestroy(gexec2[g]);
-        if (graph2[g]) (void)hipGraphDestroy(graph2[g]);
-    }
 
     FH_HIP(hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));

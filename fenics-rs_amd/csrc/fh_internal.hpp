@@ -1,5 +1,6 @@
 // fh_internal.hpp — device-resident objects behind the opaque C handles.
 #pragma once
+#include <map>
 #include <memory>
 
 #include "fh_common.hpp"
@@ -55,6 +56,20 @@ struct Adjacency {
     DBuf<uint8_t> slots;   // [entries][K] position in the pattern row
 };
 
+// A solve's workspace and its captured iteration graphs (FP64 and FP32
+// operator), kept on the matrix for the next solve with the same kernel
+// configuration: repeated solves of one operator skip the allocation and the
+// graph capture.  Defined in fh_cg.hip.
+struct CgCache {
+    DBuf<uint8_t> ws;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    CgCache() = default;
+    CgCache(const CgCache&) = delete;
+    CgCache& operator=(const CgCache&) = delete;
+    ~CgCache();
+};
+
 struct MeshP2 {
     uint32_t nedges = 0;
     DBuf<uint32_t> cell_dofs;  // [nc][10]
@@ -102,6 +117,8 @@ struct fh_matrix {
                              // the export leaves out the off-diagonal components
     fh::DBuf<uint32_t> fixed_count;  // duplicates in the prescribed list (diag value)
     uint64_t skipped_cells = 0;      // cells the assembly skipped (|det J| < 1e-14)
+    // solve workspaces and graphs by kernel configuration (fh_cg.hip run_cg)
+    std::map<uint64_t, std::unique_ptr<fh::CgCache>> cg_cache;
 };
 
 namespace fh {

@@ -392,3 +392,37 @@ def test_calls_from_two_threads_match_sequential_calls():
     for a, b in zip(seq, out):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+def test_repeated_solves_reuse_the_kept_workspace_bitwise():
+    """A solve keeps its workspace and iteration graphs on the matrix; later
+    solves of the same operator in the same configuration replay them.  Every
+    repeat (with solves in other precisions in between) returns the bits of a
+    solve of a freshly assembled copy of the operator, with the same
+    iteration counts."""
+    import torch
+    m = fh.box_beam(10.0, 2.0, 2.0, 40, 8, 8)
+    mat = fh.ElasticMaterial(210e9, 0.3)
+    bc = fh.clamp_dofs_x0(m.vertices)
+
+    def system():
+        K = fh.assemble_elasticity_stiffness_3d(m, mat)
+        b = fh.assemble_elasticity_body_force_3d(m, [0.0, 0.0, -7.85e4])
+        return fh.apply_vector_dirichlet_bc(K, b, bc)
+
+    Kb, bb = system()
+    bt = torch.tensor(bb, device="cuda:0")
+    runs = {}
+    for prec in (fh.FP64_TRUE, fh.MIXED, fh.FP64_TRUE, fh.FP64, fh.MIXED, fh.FP64_TRUE):
+        x = torch.empty_like(bt)
+        st = Kb.solve_dev(bt, x, tol=1e-10, max_iter=20000, precision=prec)
+        runs.setdefault(prec, []).append((x.cpu().numpy().view(np.uint64), st.iterations))
+    for prec, got in runs.items():
+        K2, b2 = system()
+        x2 = torch.empty_like(bt)
+        st2 = K2.solve_dev(torch.tensor(b2, device="cuda:0"), x2, tol=1e-10, max_iter=20000,
+                           precision=prec)
+        ref = x2.cpu().numpy().view(np.uint64)
+        for bits, its in got:
+            assert its == st2.iterations
+            assert np.array_equal(bits, ref), prec
