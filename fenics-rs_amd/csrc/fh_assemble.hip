@@ -123,7 +123,8 @@ __global__ __launch_bounds__(256) void k_asm_p1_tet(
 //
 // 1. k_elas_cell_geom, one thread per cell: the cell's P1 gradients
 //    g[0..3][0..2], its volume and, with a per-cell E, its Lame constants,
-//    into a 128-byte record (vol < 0 marks a cell the reference skips,
+//    into a 128-byte record of four 32-byte quarters, quarter b = (g[b], X_b)
+//    with X = (vol, lambda, mu, 0) (vol < 0 marks a cell the reference skips,
 //    |det J| < 1e-14).  The geometry is computed once per cell instead of
 //    once per (row, cell) pair.
 // 2. k_asm_elasticity_rows: the 3x3 blocks of each row vertex v (local index
@@ -163,15 +164,15 @@ __global__ __launch_bounds__(256) void k_elas_cell_geom(const double* __restrict
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) o[3 * b + i] = g[b][i];
-        o[12] = (1.0 / 6.0) * fabs(m.det);
+            for (int i = 0; i < 3; ++i) o[4 * b + i] = g[b][i];
+        o[3] = (1.0 / 6.0) * fabs(m.det);
         if (EC) {
             const double ec = Ecell[c];
-            o[13] = lame_lambda(ec, nu);
-            o[14] = lame_mu(ec, nu);
+            o[7] = lame_lambda(ec, nu);
+            o[11] = lame_mu(ec, nu);
         }
     } else {
-        o[12] = -1.0;
+        o[3] = -1.0;
     }
     // the block's records are one contiguous range: stage them in LDS and
     // store them as consecutive 16-byte pieces
@@ -215,51 +216,78 @@ __device__ __forceinline__ uint32_t elas_acc_index(uint32_t r, uint32_t kk, uint
 }
 __host__ __device__ inline uint32_t elas_acc_size(uint32_t nk) { return 9 * ((nk + 3) / 4) * 64; }
 
-// what a lane needs of one incident cell: the row's gradient, the column
-// vertex's gradient, vol, the Lame constants and the column's block slot,
-// plus the index entry and slot of the cell this record takes after it
+// Lane q of a row's four loads quarter q of an incident cell's record (two
+// 16-byte loads: g[q] and X_q), so the quad reads the whole 128-byte record
+// as one line; the row's own gradient g[a] then comes from lane a of the
+// quad (ds_bpermute) and vol, lambda, mu from lanes 0-2 (DPP broadcasts).
+// The record also holds the column's block slot and the index entry and slot
+// of the cell it takes after this one.
 struct ElasRec {
-    double ga[3], gb[3], vol, lam, mu;
-    uint32_t k, nen, nk;
+    double w[4];
+    uint32_t a, k, nen, nk;
 };
 template <bool EC>
 __device__ __forceinline__ void rec_load(const double* __restrict__ G, uint32_t en, uint32_t kq,
-                                         uint32_t q, double lambda0, double mu0, ElasRec& r) {
-    const double* gp = G + (uint64_t)(en / 4) * ELAS_GS;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        r.ga[i] = gp[3 * (en % 4) + i];
-        r.gb[i] = gp[3 * q + i];
-    }
-    r.vol = gp[12];
-    r.lam = EC ? gp[13] : lambda0;
-    r.mu = EC ? gp[14] : mu0;
+                                         uint32_t q, ElasRec& r) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2* gp = reinterpret_cast<const d2*>(G + (uint64_t)(en / 4) * ELAS_GS + 4 * q);
+    const d2 u = gp[0], v = gp[1];
+    r.w[0] = u.x;
+    r.w[1] = u.y;
+    r.w[2] = v.x;
+    r.w[3] = v.y;
+    r.a = en % 4;
     r.k = kq;
 }
-// first fill: data of cell e, index of cell e + 3 (positions past the row's
-// last cell re-read the last one and are never used)
+// value v of lane src (an absolute lane index) of the wave
+__device__ __forceinline__ double lane_read(double v, uint32_t src) {
+    const int lo = __builtin_amdgcn_ds_bpermute((int)(src << 2), __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute((int)(src << 2), __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+// value v of lane P of each quad, to every lane of the quad (DPP quad_perm)
+template <int P>
+__device__ __forceinline__ double quad_bcast(double v) {
+    constexpr int ctrl = P | P << 2 | P << 4 | P << 6;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// records in flight per lane: LDS holds the kernel to two waves per SIMD, so
+// registers are spent on depth instead
+constexpr int ELAS_RING = 6;
+// first fill: data of cell e, index of cell e + ELAS_RING (positions past
+// the row's last cell re-read the last one and are never used)
 template <bool EC>
 __device__ __forceinline__ void rec_start(const double* __restrict__ G,
                                           const uint32_t* __restrict__ aent,
                                           const uint8_t* __restrict__ slots, uint32_t e,
-                                          uint32_t last, uint32_t q, double lambda0, double mu0,
-                                          ElasRec& r) {
-    const uint32_t ec = min(e, last), en = min(e + 3, last);
-    rec_load<EC>(G, aent[ec], slots[(uint64_t)ec * 4 + q], q, lambda0, mu0, r);
+                                          uint32_t last, uint32_t q, ElasRec& r) {
+    const uint32_t ec = min(e, last), en = min(e + ELAS_RING, last);
+    rec_load<EC>(G, aent[ec], slots[(uint64_t)ec * 4 + q], q, r);
     r.nen = aent[en];
     r.nk = slots[(uint64_t)en * 4 + q];
 }
 // add cell e's block into the accumulator (the nine old values are read
 // first, so the LDS round trip overlaps the arithmetic), then load the data
-// of cell e + 3 and the index of cell e + 6
+// of cell e + ELAS_RING and the index of cell e + 2 ELAS_RING.  The four
+// lanes of a row run the same steps, so the quad exchanges below read active
+// lanes only.
 template <bool EC>
 __device__ __forceinline__ void rec_step(const double* __restrict__ G,
                                          const uint32_t* __restrict__ aent,
                                          const uint8_t* __restrict__ slots, uint32_t e,
-                                         uint32_t last, uint32_t q, double lambda0, double mu0,
-                                         uint32_t k0, uint32_t kmax, double* acc, uint32_t racc,
-                                         ElasRec& r) {
-    if (!(r.vol < 0.0) && r.k >= k0 && r.k < kmax) {
+                                         uint32_t last, uint32_t q, uint32_t lane, double lambda0,
+                                         double mu0, uint32_t k0, uint32_t kmax, double* acc,
+                                         uint32_t racc, ElasRec& r) {
+    const uint32_t src = (lane & ~3u) | r.a;
+    double ga[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ga[i] = lane_read(r.w[i], src);
+    const double vol = quad_bcast<0>(r.w[3]);
+    const double lam = EC ? quad_bcast<1>(r.w[3]) : lambda0;
+    const double mu = EC ? quad_bcast<2>(r.w[3]) : mu0;
+    if (!(vol < 0.0) && r.k >= k0 && r.k < kmax) {
         // components j of one block are 64 doubles apart
         double* ab = acc + elas_acc_index(racc, r.k - k0, 0);
         double old[9];
@@ -270,10 +298,10 @@ __device__ __forceinline__ void rec_step(const double* __restrict__ G,
 #pragma unroll
             for (int be = 0; be < 3; ++be)
                 ab[(al * 3 + be) * 64] =
-                    old[al * 3 + be] + elas_entry(r.ga, r.gb, al, be, r.lam, r.mu, r.vol);
+                    old[al * 3 + be] + elas_entry(ga, r.w, al, be, lam, mu, vol);
     }
-    const uint32_t en = min(e + 6, last);
-    rec_load<EC>(G, r.nen, r.nk, q, lambda0, mu0, r);
+    const uint32_t en = min(e + 2 * ELAS_RING, last);
+    rec_load<EC>(G, r.nen, r.nk, q, r);
     r.nen = aent[en];
     r.nk = slots[(uint64_t)en * 4 + q];
 }
@@ -307,24 +335,27 @@ __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
             __syncthreads();
             if (e0 < e1) {
                 const uint32_t last = e1 - 1;
-                // three records in rotation: record i holds the data of the cell
-                // it processes next and the index of the cell three after that,
-                // so data arrive three steps and indices six steps ahead of use
-                ElasRec r0, r1, r2;
-                rec_start<EC>(G, aent, slots, e0, last, q, lambda0, mu0, r0);
-                rec_start<EC>(G, aent, slots, e0 + 1, last, q, lambda0, mu0, r1);
-                rec_start<EC>(G, aent, slots, e0 + 2, last, q, lambda0, mu0, r2);
+                // ELAS_RING records in rotation: record i holds the data of the
+                // cell it processes next and the index of the cell ELAS_RING
+                // after that, so data arrive ELAS_RING steps and indices twice
+                // that ahead of use
+                ElasRec r[ELAS_RING];
+#pragma unroll
+                for (int i = 0; i < ELAS_RING; ++i)
+                    rec_start<EC>(G, aent, slots, e0 + i, last, q, r[i]);
                 const uint32_t kmax = nk + k0;
                 uint32_t e = e0;
-                for (; e + 2 < e1; e += 3) {
-                    rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
-                    rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
-                    rec_step<EC>(G, aent, slots, e + 2, last, q, lambda0, mu0, k0, kmax, acc, racc, r2);
+                for (; e + ELAS_RING - 1 < e1; e += ELAS_RING) {
+#pragma unroll
+                    for (int i = 0; i < ELAS_RING; ++i)
+                        rec_step<EC>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
+                                     acc, racc, r[i]);
                 }
-                if (e < e1)
-                    rec_step<EC>(G, aent, slots, e, last, q, lambda0, mu0, k0, kmax, acc, racc, r0);
-                if (e + 1 < e1)
-                    rec_step<EC>(G, aent, slots, e + 1, last, q, lambda0, mu0, k0, kmax, acc, racc, r1);
+#pragma unroll
+                for (int i = 0; i < ELAS_RING - 1; ++i)
+                    if (e + i < e1)
+                        rec_step<EC>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
+                                     acc, racc, r[i]);
             }
             __syncthreads();
             // this pass's blocks: lane (wpp, wr) stores the component pair wpp of

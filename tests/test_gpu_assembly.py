@@ -246,6 +246,29 @@ def test_unstructured_mesh_bitwise(dim):
         assert np.array_equal(bits(bc), bits(bco))
 
 
+def test_high_valence_vertex_bitwise():
+    """A vertex shared by every cell: the tetrahedra (centre, hull triangle)
+    of 64 points on a sphere around it.  The centre's row holds 65 blocks,
+    more than the 48 the elasticity row kernel sums in LDS at once, so the
+    kernel covers it in two passes; the assembled matrices stay bitwise the
+    oracle's."""
+    from scipy.spatial import ConvexHull
+    rng = np.random.default_rng(11)
+    p = rng.normal(size=(64, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    tri = ConvexHull(p).simplices
+    V = np.vstack([np.zeros((1, 3)), p]).astype(np.float64)
+    C = np.hstack([np.zeros((len(tri), 1), dtype=np.uint64), tri.astype(np.uint64) + 1])
+    om = O.Mesh(V, C)
+    gm = fh.Mesh.from_arrays(V, C)
+    same_csc(fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.3)),
+             O.assemble_elasticity(om, 7e10, 0.3))
+    ec = rng.uniform(1e9, 5e9, len(C))
+    same_csc(fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.3), ec),
+             O.assemble_elasticity(om, ec, 0.3))
+    same_csc(fh.assemble_stiffness(gm), O.assemble_stiffness(om))
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_degenerate_cells_are_skipped_and_counted(dim):
     """jacobian.rs:29-31 / poisson.rs:42-45: a cell with |det J| < 1e-14 adds
