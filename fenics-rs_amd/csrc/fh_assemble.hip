@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "fh_geom.hpp"
@@ -46,10 +47,10 @@ __device__ __forceinline__ void init_row(double* vals, const RowCtx& rc) {
 }
 
 // poisson.rs:33-66 (kind 0: stiffness) and mass.rs:16-41 (kind 1: mass), P1 triangles
-template <int KIND>
+template <int KIND, typename S>
 __global__ __launch_bounds__(256) void k_asm_p1_tri(
     const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ aent, const S* __restrict__ slots,
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
     double* __restrict__ vals) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,10 +84,10 @@ __global__ __launch_bounds__(256) void k_asm_p1_tri(
 }
 
 // poisson.rs:129-162 (kind 0) and mass.rs:53-79 (kind 1), P1 tetrahedra
-template <int KIND>
+template <int KIND, typename S>
 __global__ __launch_bounds__(256) void k_asm_p1_tet(
     const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ aent, const S* __restrict__ slots,
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
     double* __restrict__ vals) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,10 +259,10 @@ __device__ __forceinline__ double quad_bcast(double v) {
 constexpr int ELAS_RING = 6;
 // first fill: data of cell e, index of cell e + ELAS_RING (positions past
 // the row's last cell re-read the last one and are never used)
-template <bool EC>
+template <bool EC, typename S>
 __device__ __forceinline__ void rec_start(const double* __restrict__ G,
                                           const uint32_t* __restrict__ aent,
-                                          const uint8_t* __restrict__ slots, uint32_t e,
+                                          const S* __restrict__ slots, uint32_t e,
                                           uint32_t last, uint32_t q, ElasRec& r) {
     const uint32_t ec = min(e, last), en = min(e + ELAS_RING, last);
     rec_load<EC>(G, aent[ec], slots[(uint64_t)ec * 4 + q], q, r);
@@ -273,10 +274,10 @@ __device__ __forceinline__ void rec_start(const double* __restrict__ G,
 // of cell e + ELAS_RING and the index of cell e + 2 ELAS_RING.  The four
 // lanes of a row run the same steps, so the quad exchanges below read active
 // lanes only.
-template <bool EC>
+template <bool EC, typename S>
 __device__ __forceinline__ void rec_step(const double* __restrict__ G,
                                          const uint32_t* __restrict__ aent,
-                                         const uint8_t* __restrict__ slots, uint32_t e,
+                                         const S* __restrict__ slots, uint32_t e,
                                          uint32_t last, uint32_t q, uint32_t lane, double lambda0,
                                          double mu0, uint32_t k0, uint32_t kmax, double* acc,
                                          uint32_t racc, ElasRec& r) {
@@ -306,10 +307,10 @@ __device__ __forceinline__ void rec_step(const double* __restrict__ G,
     r.nk = slots[(uint64_t)en * 4 + q];
 }
 
-template <bool EC>
+template <bool EC, typename S>
 __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
     const double* __restrict__ G, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ aent, const S* __restrict__ slots,
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
     double lambda0, double mu0, uint32_t wp, uint32_t nunits, uint32_t per,
     double* __restrict__ vals) {
@@ -342,19 +343,19 @@ __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
                 ElasRec r[ELAS_RING];
 #pragma unroll
                 for (int i = 0; i < ELAS_RING; ++i)
-                    rec_start<EC>(G, aent, slots, e0 + i, last, q, r[i]);
+                    rec_start<EC, S>(G, aent, slots, e0 + i, last, q, r[i]);
                 const uint32_t kmax = nk + k0;
                 uint32_t e = e0;
                 for (; e + ELAS_RING - 1 < e1; e += ELAS_RING) {
 #pragma unroll
                     for (int i = 0; i < ELAS_RING; ++i)
-                        rec_step<EC>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
+                        rec_step<EC, S>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
                                      acc, racc, r[i]);
                 }
 #pragma unroll
                 for (int i = 0; i < ELAS_RING - 1; ++i)
                     if (e + i < e1)
-                        rec_step<EC>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
+                        rec_step<EC, S>(G, aent, slots, e + i, last, q, lane, lambda0, mu0, k0, kmax,
                                      acc, racc, r[i]);
             }
             __syncthreads();
@@ -378,9 +379,10 @@ __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
 
 // mass.rs:86-126: M_{(v,al),(b,al)} += density * vol * (1/10 if a == b else
 // 1/20) for every incident cell, the same scalar mass on each component.
+template <typename S>
 __global__ __launch_bounds__(256) void k_asm_elasticity_mass(
     const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ aent, const S* __restrict__ slots,
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nv,
     double density, double* __restrict__ vals) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -413,9 +415,10 @@ __global__ __launch_bounds__(256) void k_asm_elasticity_mass(
 struct P2MassTab {
     double m[100];
 };
+template <typename S>
 __global__ __launch_bounds__(256) void k_asm_p2_vec_mass(
     const double* __restrict__ V, const uint32_t* __restrict__ C, const uint32_t* __restrict__ aptr,
-    const uint32_t* __restrict__ aent, const uint8_t* __restrict__ slots,
+    const uint32_t* __restrict__ aent, const S* __restrict__ slots,
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ rlen, uint32_t nd,
     double density, P2MassTab tab, double* __restrict__ vals) {
     const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,8 +586,8 @@ __global__ __launch_bounds__(64) void k_p2_elem_f64(const double* __restrict__ V
 // float) loads and two small ones.  Rows are summed in LDS (p2_acc_index,
 // banks as in the elasticity kernel) in ascending cell order per slot, and
 // written once (every SELL entry, padding included; with vals64 also as
-// doubles).  A slot map of u8 bounds a row at 255 entries, so one pass
-// always covers the row.
+// doubles).  The accumulator holds wp blocks per row (512: 64 KB of LDS for
+// doubles); a wider row is done in passes over ranges of wp blocks.
 __device__ __forceinline__ uint32_t p2_acc_index(uint32_t r, uint32_t k) {
     return ((k / 4) * 2 + r / 8) * 32 + (r % 8) * 4 + k % 4;
 }
@@ -592,6 +595,15 @@ __device__ __forceinline__ bool p2_skipped(double v) {
     return (uint64_t)__double_as_longlong(v) == P2_SKIP_BITS;
 }
 __device__ __forceinline__ bool p2_skipped(float) { return false; }
+
+// two consecutive slots in one load, as (first | second << 16)
+__device__ __forceinline__ uint32_t slot_pair(const uint8_t* p) {
+    const uint32_t v = *reinterpret_cast<const uint16_t*>(p);
+    return (v & 0xffu) | (v >> 8) << 16;
+}
+__device__ __forceinline__ uint32_t slot_pair(const uint16_t* p) {
+    return *reinterpret_cast<const uint32_t*>(p);
+}
 
 // lane q's part of one incident cell's element-matrix row: entries (2q,
 // 2q+1) and, for q < 2, 8 + q, their slots, and the index entry of the cell
@@ -601,41 +613,47 @@ struct P2Rec {
     T v0, v1, v2;
     uint32_t s01, s2, nen, ne;
 };
-template <typename T>
-__device__ __forceinline__ void p2_load(const T* __restrict__ Ke, const uint8_t* __restrict__ slots,
+template <typename T, typename S>
+__device__ __forceinline__ void p2_load(const T* __restrict__ Ke, const S* __restrict__ slots,
                                         uint32_t en, uint32_t e, uint32_t q, P2Rec<T>& r) {
     typedef T t2 __attribute__((ext_vector_type(2)));
     const T* kr = Ke + (uint64_t)(en / 10) * 100 + (en % 10) * 10;
     const t2 v = *reinterpret_cast<const t2*>(kr + 2 * q);
     r.v0 = v.x;
     r.v1 = v.y;
-    r.s01 = *reinterpret_cast<const uint16_t*>(slots + (uint64_t)e * 10 + 2 * q);
+    r.s01 = slot_pair(slots + (uint64_t)e * 10 + 2 * q);
     if (q < 2) {
         r.v2 = kr[8 + q];
         r.s2 = slots[(uint64_t)e * 10 + 8 + q];
     }
 }
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ void p2_start(const T* __restrict__ Ke, const uint32_t* __restrict__ aent,
-                                         const uint8_t* __restrict__ slots, uint32_t e,
+                                         const S* __restrict__ slots, uint32_t e,
                                          uint32_t last, uint32_t q, P2Rec<T>& r) {
     const uint32_t ec = min(e, last);
     p2_load(Ke, slots, aent[ec], ec, q, r);
     r.ne = min(e + 3, last);
     r.nen = aent[r.ne];
 }
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ void p2_step(const T* __restrict__ Ke, const uint32_t* __restrict__ aent,
-                                        const uint8_t* __restrict__ slots, uint32_t e,
-                                        uint32_t last, uint32_t q, uint32_t r, T* acc, P2Rec<T>& x) {
+                                        const S* __restrict__ slots, uint32_t e,
+                                        uint32_t last, uint32_t q, uint32_t r, uint32_t k0,
+                                        uint32_t nk, T* acc, P2Rec<T>& x) {
     if (!p2_skipped(x.v0)) {
-        T* a0 = acc + p2_acc_index(r, x.s01 & 255u);
-        T* a1 = acc + p2_acc_index(r, x.s01 >> 8);
-        const T o0 = *a0, o1 = *a1;
-        *a0 = o0 + x.v0;
-        *a1 = o1 + x.v1;
-        if (q < 2) {
-            T* a2 = acc + p2_acc_index(r, x.s2);
+        // slots of this pass's blocks [k0, k0 + nk) only
+        const uint32_t ka = (x.s01 & 0xffffu) - k0, kb = (x.s01 >> 16) - k0;  // slot_pair
+        if (ka < nk) {
+            T* a0 = acc + p2_acc_index(r, ka);
+            *a0 = *a0 + x.v0;
+        }
+        if (kb < nk) {
+            T* a1 = acc + p2_acc_index(r, kb);
+            *a1 = *a1 + x.v1;
+        }
+        if (q < 2 && x.s2 - k0 < nk) {
+            T* a2 = acc + p2_acc_index(r, x.s2 - k0);
             *a2 = *a2 + x.v2;
         }
     }
@@ -644,11 +662,11 @@ __device__ __forceinline__ void p2_step(const T* __restrict__ Ke, const uint32_t
     x.nen = aent[x.ne];
 }
 
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(64) void k_gather_p2_rows(
     const T* __restrict__ Ke, const uint32_t* __restrict__ aptr, const uint32_t* __restrict__ aent,
-    const uint8_t* __restrict__ slots, const uint32_t* __restrict__ soff,
-    const uint32_t* __restrict__ rlen, uint32_t nd, uint32_t nunits, uint32_t per,
+    const S* __restrict__ slots, const uint32_t* __restrict__ soff,
+    const uint32_t* __restrict__ rlen, uint32_t nd, uint32_t wp, uint32_t nunits, uint32_t per,
     T* __restrict__ vals, double* __restrict__ vals64) {
     extern __shared__ __align__(16) unsigned char smem_p2[];
     T* acc = reinterpret_cast<T*>(smem_p2);
@@ -661,37 +679,40 @@ __global__ __launch_bounds__(64) void k_gather_p2_rows(
         const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
         const bool live = row < nd;
         const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
-        const uint32_t accn = ((w + 3) / 4) * 64;
-        for (uint32_t i = lane; i < accn; i += SLICE) acc[i] = (T)-0.0;
-        __syncthreads();
-        if (e0 < e1) {
-            // three records in rotation, as in k_asm_elasticity_rows: data three
-            // cells and indices six cells ahead of use
-            const uint32_t last = e1 - 1;
-            P2Rec<T> r0, r1, r2;
-            p2_start(Ke, aent, slots, e0, last, q, r0);
-            p2_start(Ke, aent, slots, e0 + 1, last, q, r1);
-            p2_start(Ke, aent, slots, e0 + 2, last, q, r2);
-            uint32_t e = e0;
-            for (; e + 2 < e1; e += 3) {
-                p2_step(Ke, aent, slots, e, last, q, r, acc, r0);
-                p2_step(Ke, aent, slots, e + 1, last, q, r, acc, r1);
-                p2_step(Ke, aent, slots, e + 2, last, q, r, acc, r2);
-            }
-            if (e < e1) p2_step(Ke, aent, slots, e, last, q, r, acc, r0);
-            if (e + 1 < e1) p2_step(Ke, aent, slots, e + 1, last, q, r, acc, r1);
-        }
-        __syncthreads();
-        // lane (kq, wr) stores blocks kq, kq + 4, ... of row wr; zeros past its length
+        // the row's writer lane (kq, wr) stores blocks kq, kq + 4, ... of row wr
         const uint32_t wr = lane % 16, wrow = sl * SLICE + h * 16 + wr;
         const uint32_t wlen = wrow < nd ? rlen[wrow] : 0;
-        for (uint32_t k = lane / 16; k < w; k += 4) {
-            const uint64_t ix = (uint64_t)base + (uint64_t)k * SLICE + h * 16 + wr;
-            const T v = k < wlen ? acc[p2_acc_index(wr, k)] : (T)0;
-            vals[ix] = v;
-            if (vals64) vals64[ix] = (double)v;
+        for (uint32_t k0 = 0; k0 < w; k0 += wp) {
+            const uint32_t nk = min(wp, w - k0), accn = ((nk + 3) / 4) * 64;
+            for (uint32_t i = lane; i < accn; i += SLICE) acc[i] = (T)-0.0;
+            __syncthreads();
+            if (e0 < e1) {
+                // three records in rotation, as in k_asm_elasticity_rows: data
+                // three cells and indices six cells ahead of use
+                const uint32_t last = e1 - 1;
+                P2Rec<T> r0, r1, r2;
+                p2_start(Ke, aent, slots, e0, last, q, r0);
+                p2_start(Ke, aent, slots, e0 + 1, last, q, r1);
+                p2_start(Ke, aent, slots, e0 + 2, last, q, r2);
+                uint32_t e = e0;
+                for (; e + 2 < e1; e += 3) {
+                    p2_step(Ke, aent, slots, e, last, q, r, k0, nk, acc, r0);
+                    p2_step(Ke, aent, slots, e + 1, last, q, r, k0, nk, acc, r1);
+                    p2_step(Ke, aent, slots, e + 2, last, q, r, k0, nk, acc, r2);
+                }
+                if (e < e1) p2_step(Ke, aent, slots, e, last, q, r, k0, nk, acc, r0);
+                if (e + 1 < e1) p2_step(Ke, aent, slots, e + 1, last, q, r, k0, nk, acc, r1);
+            }
+            __syncthreads();
+            // this pass's blocks; zeros past the row's length
+            for (uint32_t k = k0 + lane / 16; k < k0 + nk; k += 4) {
+                const uint64_t ix = (uint64_t)base + (uint64_t)k * SLICE + h * 16 + wr;
+                const T v = k < wlen ? acc[p2_acc_index(wr, k - k0)] : (T)0;
+                vals[ix] = v;
+                if (vals64) vals64[ix] = (double)v;
+            }
+            __syncthreads();  // the next pass or unit clears the accumulator
         }
-        __syncthreads();  // the next unit clears the accumulator
     }
 }
 
@@ -921,6 +942,14 @@ static P2Tab p2_table() {
     return t;
 }
 
+// calls f with the adjacency's slot map (u8, or u16 for rows past 255 blocks)
+template <class F>
+static void with_slots(const Adjacency& adj, F&& f) {
+    if (adj.slots16.p) f(adj.slots16.p);
+    else f(adj.slots.p);
+}
+#define FH_SLOT_T(sl) std::remove_const_t<std::remove_pointer_t<decltype(sl)>>
+
 // Blocks of the row kernels (one wave each, looping over the units of
 // their XCD): FH_ASM_WAVES_PER_CU (default 8) per compute unit, at most one
 // per unit.  A multiple of 8, so block b stays on XCD b % 8.
@@ -938,17 +967,21 @@ void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, double* vals64,
     const Pattern& pat = *p2.pat;
     const uint32_t nunits = pat.nslices * 4, per = row_units_per_xcd(nunits);
     if (!nunits) return;
-    const size_t accn = (size_t)((pat.max_width + 3) / 4) * 64;
-    if (f64)
-        hipLaunchKernelGGL(k_gather_p2_rows<double>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
-                           accn * sizeof(double), s, (const double*)Ke, p2.adj->ptr.p,
-                           p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
-                           p2.adj->ndofs, nunits, per, (double*)vals, nullptr);
-    else
-        hipLaunchKernelGGL(k_gather_p2_rows<float>, dim3(rows_grid(m->dev, per)), dim3(SLICE),
-                           accn * sizeof(float), s, (const float*)Ke, p2.adj->ptr.p,
-                           p2.adj->ent.p, p2.adj->slots.p, pat.slice_off.p, pat.row_len.p,
-                           p2.adj->ndofs, nunits, per, (float*)vals, vals64);
+    const uint32_t wp = std::max(1u, std::min(pat.max_width, 512u));
+    const size_t accn = (size_t)((wp + 3) / 4) * 64;
+    with_slots(*p2.adj, [&](auto* sl) {
+        using S = FH_SLOT_T(sl);
+        if (f64)
+            hipLaunchKernelGGL((k_gather_p2_rows<double, S>), dim3(rows_grid(m->dev, per)),
+                               dim3(SLICE), accn * sizeof(double), s, (const double*)Ke,
+                               p2.adj->ptr.p, p2.adj->ent.p, sl, pat.slice_off.p, pat.row_len.p,
+                               p2.adj->ndofs, wp, nunits, per, (double*)vals, nullptr);
+        else
+            hipLaunchKernelGGL((k_gather_p2_rows<float, S>), dim3(rows_grid(m->dev, per)),
+                               dim3(SLICE), accn * sizeof(float), s, (const float*)Ke,
+                               p2.adj->ptr.p, p2.adj->ent.p, sl, pat.slice_off.p, pat.row_len.p,
+                               p2.adj->ndofs, wp, nunits, per, (float*)vals, vals64);
+    });
     FH_LAUNCH_CHECK("gather_p2_rows");
 }
 
@@ -1020,14 +1053,18 @@ void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ece
     const uint32_t nunits = p.nslices * (SLICE / ELAS_RPW);
     const uint32_t per = row_units_per_xcd(nunits);
     if (!nunits) return;
-    if (ec)
-        hipLaunchKernelGGL((k_asm_elasticity_rows<true>), dim3(rows_grid(a->dev, per)), dim3(SLICE), lds, s,
-                           geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
-                           m->nv, 0.0, 0.0, wp, nunits, per, a->vals.p);
-    else
-        hipLaunchKernelGGL((k_asm_elasticity_rows<false>), dim3(rows_grid(a->dev, per)), dim3(SLICE), lds, s,
-                           geom.p, adj.ptr.p, adj.ent.p, adj.slots.p, p.slice_off.p, p.row_len.p,
-                           m->nv, lame_lambda(E, nu), lame_mu(E, nu), wp, nunits, per, a->vals.p);
+    const dim3 grid(rows_grid(a->dev, per));
+    with_slots(adj, [&](auto* sl) {
+        using S = FH_SLOT_T(sl);
+        if (ec)
+            hipLaunchKernelGGL((k_asm_elasticity_rows<true, S>), grid, dim3(SLICE), lds, s, geom.p,
+                               adj.ptr.p, adj.ent.p, sl, p.slice_off.p, p.row_len.p, m->nv, 0.0,
+                               0.0, wp, nunits, per, a->vals.p);
+        else
+            hipLaunchKernelGGL((k_asm_elasticity_rows<false, S>), grid, dim3(SLICE), lds, s,
+                               geom.p, adj.ptr.p, adj.ent.p, sl, p.slice_off.p, p.row_len.p, m->nv,
+                               lame_lambda(E, nu), lame_mu(E, nu), wp, nunits, per, a->vals.p);
+    });
     FH_LAUNCH_CHECK("asm_elasticity");
     // geom goes back to the allocator's cache; the next user of that memory is
     // ordered after these kernels on the same stream
@@ -1050,9 +1087,11 @@ fh_matrix* assemble_elasticity_mass(fh_mesh* m, double density) {
     auto pat = mesh_pattern(m);
     Adjacency& adj = mesh_adjacency(m);
     std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 3));
-    hipLaunchKernelGGL(k_asm_elasticity_mass, dim3(cdiv(m->nv, 256)), dim3(256), 0, d.stream,
-                       m->verts.p, m->cells.p, adj.ptr.p, adj.ent.p, adj.slots.p,
-                       pat->slice_off.p, pat->row_len.p, m->nv, density, a->vals.p);
+    with_slots(adj, [&](auto* sl) {
+        hipLaunchKernelGGL((k_asm_elasticity_mass<FH_SLOT_T(sl)>), dim3(cdiv(m->nv, 256)),
+                           dim3(256), 0, d.stream, m->verts.p, m->cells.p, adj.ptr.p, adj.ent.p,
+                           sl, pat->slice_off.p, pat->row_len.p, m->nv, density, a->vals.p);
+    });
     FH_LAUNCH_CHECK("asm_elasticity_mass");
     FH_HIP(hipStreamSynchronize(d.stream));
     a->comp_diag = true;
@@ -1124,10 +1163,12 @@ fh_matrix* assemble_elasticity_mass_p2(int dev, uint64_t n_nodes, const double* 
     build_slots(C.p, 10, *adj, *pat, s);
     std::unique_ptr<fh_matrix> a(new_matrix(dev, pat, 3));
     if (n_nodes)
-        hipLaunchKernelGGL(k_asm_p2_vec_mass, dim3(cdiv(n_nodes, 256)), dim3(256), 0, s, V.p, C.p,
-                           adj->ptr.p, adj->ent.p, adj->slots.p, pat->slice_off.p,
-                           pat->row_len.p, (uint32_t)n_nodes, density, p2_mass_table(),
-                           a->vals.p);
+        with_slots(*adj, [&](auto* sl) {
+            hipLaunchKernelGGL((k_asm_p2_vec_mass<FH_SLOT_T(sl)>), dim3(cdiv(n_nodes, 256)),
+                               dim3(256), 0, s, V.p, C.p, adj->ptr.p, adj->ent.p, sl,
+                               pat->slice_off.p, pat->row_len.p, (uint32_t)n_nodes, density,
+                               p2_mass_table(), a->vals.p);
+        });
     FH_LAUNCH_CHECK("asm_p2_vec_mass");
     FH_HIP(hipStreamSynchronize(s));
     a->comp_diag = true;
@@ -1290,25 +1331,20 @@ fh_matrix* assemble_p1(fh_mesh* m, int kind) {
     Adjacency& adj = mesh_adjacency(m);
     std::unique_ptr<fh_matrix> a(new_matrix(m->dev, pat, 1));
     const dim3 g(cdiv(m->nv, 256)), b(256);
-    if (m->dim == 2) {
-        if (kind == 0)
-            hipLaunchKernelGGL(k_asm_p1_tri<0>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
-                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
-                               a->vals.p);
-        else
-            hipLaunchKernelGGL(k_asm_p1_tri<1>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
-                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
-                               a->vals.p);
-    } else {
-        if (kind == 0)
-            hipLaunchKernelGGL(k_asm_p1_tet<0>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
-                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
-                               a->vals.p);
-        else
-            hipLaunchKernelGGL(k_asm_p1_tet<1>, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p,
-                               adj.ent.p, adj.slots.p, pat->slice_off.p, pat->row_len.p, m->nv,
-                               a->vals.p);
-    }
+    with_slots(adj, [&](auto* sl) {
+        using S = FH_SLOT_T(sl);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, g, b, 0, s, m->verts.p, m->cells.p, adj.ptr.p, adj.ent.p,
+                               sl, pat->slice_off.p, pat->row_len.p, m->nv, a->vals.p);
+        };
+        if (m->dim == 2) {
+            if (kind == 0) go(k_asm_p1_tri<0, S>);
+            else go(k_asm_p1_tri<1, S>);
+        } else {
+            if (kind == 0) go(k_asm_p1_tet<0, S>);
+            else go(k_asm_p1_tet<1, S>);
+        }
+    });
     FH_LAUNCH_CHECK("asm_p1");
     a->skipped_cells = mesh_degenerate_cells(m);
     return a.release();

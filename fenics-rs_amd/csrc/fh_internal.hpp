@@ -53,7 +53,8 @@ struct Adjacency {
     int K = 0;
     DBuf<uint32_t> ptr;    // [ndofs+1]
     DBuf<uint32_t> ent;    // cell * K + local
-    DBuf<uint8_t> slots;   // [entries][K] position in the pattern row
+    DBuf<uint8_t> slots;     // [entries][K] position in the pattern row (rows of <= 255 blocks)
+    DBuf<uint16_t> slots16;  // the same as u16, instead, for rows of 256-65535 blocks
 };
 
 // A solve's workspace and its captured iteration graphs (FP64 and FP32

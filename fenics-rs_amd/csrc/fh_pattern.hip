@@ -14,6 +14,7 @@
 //   4. the SELL-64 layout (64 consecutive rows per slice, column-major inside
 //      the slice, slice width = longest row).
 #include <algorithm>
+#include <type_traits>
 
 #include "fh_internal.hpp"
 
@@ -100,11 +101,11 @@ __global__ void k_row_pattern(const uint32_t* __restrict__ cd, const uint32_t* _
     if (!FILL) len[v] = cnt;
 }
 
-template <int K>
+template <int K, typename S>
 __global__ void k_slots(const uint32_t* __restrict__ cd, const uint32_t* __restrict__ aptr,
                         const uint32_t* __restrict__ aent, uint32_t nd,
                         const uint32_t* __restrict__ rptr, const uint32_t* __restrict__ cols,
-                        uint8_t* __restrict__ slots) {
+                        S* __restrict__ slots) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nd) return;
     const uint32_t r0 = rptr[v], r1 = rptr[v + 1];
@@ -119,7 +120,7 @@ __global__ void k_slots(const uint32_t* __restrict__ cd, const uint32_t* __restr
                 if (cols[mid] < d) lo = mid + 1;
                 else hi = mid;
             }
-            slots[(uint64_t)e * K + b] = (uint8_t)(lo - r0);
+            slots[(uint64_t)e * K + b] = (S)(lo - r0);
         }
     }
 }
@@ -201,14 +202,14 @@ std::shared_ptr<Pattern> build_pattern(int dev, const uint32_t* cd, int K, Adjac
     hipLaunchKernelGGL(k_sell_cols, dim3(cdiv((size_t)p->nslices * SLICE, 256)), dim3(256), 0, s,
                        p->row_ptr.p, p->cols.p, p->slice_off.p, n, p->nslices, p->sell_cols.p);
     FH_LAUNCH_CHECK("sell_cols");
-    // widest row (slot indices are stored as u8)
+    // widest row (slot indices are stored as u8, or as u16 past 255 blocks)
     std::vector<uint32_t> hw(p->nslices);
     d2h(hw.data(), w.p, p->nslices, s);
     FH_HIP(hipStreamSynchronize(s));
     uint32_t mw = 0;
     for (uint32_t x : hw) mw = std::max(mw, x / SLICE);
     p->max_width = mw;
-    if (mw > 255) throw Error(FH_ERR_UNSUPPORTED, "row longer than 255 blocks");
+    if (mw > 65535) throw Error(FH_ERR_UNSUPPORTED, "row longer than 65535 blocks");
     return p;
 }
 
@@ -217,21 +218,33 @@ void build_slots(const uint32_t* cd, int K, Adjacency& adj, const Pattern& pat, 
     FH_HIP(hipMemcpyAsync(&nent, adj.ptr.p + adj.ndofs, 4, hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
     nent &= 0xffffffffull;
-    adj.slots.alloc(pat.dev, std::max<uint64_t>(nent * K, 1));
     const unsigned g = cdiv(adj.ndofs, 64);
-    switch (K) {
-        case 3:
-            hipLaunchKernelGGL(k_slots<3>, dim3(g), dim3(64), 0, s, cd, adj.ptr.p, adj.ent.p,
-                               adj.ndofs, pat.row_ptr.p, pat.cols.p, adj.slots.p);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_slots<4>, dim3(g), dim3(64), 0, s, cd, adj.ptr.p, adj.ent.p,
-                               adj.ndofs, pat.row_ptr.p, pat.cols.p, adj.slots.p);
-            break;
-        case 10:
-            hipLaunchKernelGGL(k_slots<10>, dim3(g), dim3(64), 0, s, cd, adj.ptr.p, adj.ent.p,
-                               adj.ndofs, pat.row_ptr.p, pat.cols.p, adj.slots.p);
-            break;
+    auto launch = [&](auto* out) {
+        using S = std::remove_pointer_t<decltype(out)>;
+        switch (K) {
+            case 3:
+                hipLaunchKernelGGL((k_slots<3, S>), dim3(g), dim3(64), 0, s, cd, adj.ptr.p,
+                                   adj.ent.p, adj.ndofs, pat.row_ptr.p, pat.cols.p, out);
+                break;
+            case 4:
+                hipLaunchKernelGGL((k_slots<4, S>), dim3(g), dim3(64), 0, s, cd, adj.ptr.p,
+                                   adj.ent.p, adj.ndofs, pat.row_ptr.p, pat.cols.p, out);
+                break;
+            case 10:
+                hipLaunchKernelGGL((k_slots<10, S>), dim3(g), dim3(64), 0, s, cd, adj.ptr.p,
+                                   adj.ent.p, adj.ndofs, pat.row_ptr.p, pat.cols.p, out);
+                break;
+        }
+    };
+    // u8 slots while every row fits (half the bytes the assembly kernels read)
+    if (pat.max_width > 255) {
+        adj.slots.reset();
+        adj.slots16.alloc(pat.dev, std::max<uint64_t>(nent * K, 1));
+        launch(adj.slots16.p);
+    } else {
+        adj.slots16.reset();
+        adj.slots.alloc(pat.dev, std::max<uint64_t>(nent * K, 1));
+        launch(adj.slots.p);
     }
     FH_LAUNCH_CHECK("slots");
 }

@@ -248,13 +248,14 @@ def test_unstructured_mesh_bitwise(dim):
 
 def test_high_valence_vertex_bitwise():
     """A vertex shared by every cell: the tetrahedra (centre, hull triangle)
-    of 64 points on a sphere around it.  The centre's row holds 65 blocks,
-    more than the 48 the elasticity row kernel sums in LDS at once, so the
-    kernel covers it in two passes; the assembled matrices stay bitwise the
-    oracle's."""
+    of 180 points on a sphere around it.  The centre's row holds 181 blocks
+    (P1) and 895 (P2): more than the 255 a u8 slot map could index, and more
+    than the 48 (elasticity) and 512 (P2 gather) blocks the row kernels sum in
+    LDS at once, so both kernels cover it in passes.  The assembled matrices
+    stay bitwise the oracle's."""
     from scipy.spatial import ConvexHull
     rng = np.random.default_rng(11)
-    p = rng.normal(size=(64, 3))
+    p = rng.normal(size=(180, 3))
     p /= np.linalg.norm(p, axis=1, keepdims=True)
     tri = ConvexHull(p).simplices
     V = np.vstack([np.zeros((1, 3)), p]).astype(np.float64)
@@ -267,6 +268,10 @@ def test_high_valence_vertex_bitwise():
     same_csc(fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.3), ec),
              O.assemble_elasticity(om, ec, 0.3))
     same_csc(fh.assemble_stiffness(gm), O.assemble_stiffness(om))
+    p2 = O.P2(om)
+    assert gm.p2_info() == (p2.ndofs, p2.nedges)
+    assert np.array_equal(gm.p2_cell_dofs(), p2.cell_dofs())
+    same_csc(fh.assemble_p2_stiffness(gm, 0), p2.stiffness())
 
 
 @pytest.mark.parametrize("dim", [2, 3])
