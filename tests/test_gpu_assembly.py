@@ -272,6 +272,10 @@ def test_high_valence_vertex_bitwise():
     assert gm.p2_info() == (p2.ndofs, p2.nedges)
     assert np.array_equal(gm.p2_cell_dofs(), p2.cell_dofs())
     same_csc(fh.assemble_p2_stiffness(gm, 0), p2.stiffness())
+    # the FP32 MFMA element path through the same wide-row gather
+    _, _, v64 = fh.assemble_p2_stiffness(gm, 0).export_csc()
+    _, _, v32 = fh.assemble_p2_stiffness(gm, 1).export_csc()
+    assert np.abs(v32 - v64).max() <= 2e-6 * np.abs(v64).max()
 
 
 @pytest.mark.parametrize("dim", [2, 3])

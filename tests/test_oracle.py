@@ -383,6 +383,31 @@ def test_p2_dofmap_counts_unit_cube():
     assert np.abs(K @ np.ones(p2.ndofs)).max() < 1e-12
 
 
+def test_p2_edge_numbering_convention():
+    """P2 dofs: vertices keep their numbers; edge dofs follow ordered by the
+    number of cells around the edge (most first), ties in lexicographic
+    (min, max) vertex order; every edge's midpoint is its dof coordinate."""
+    m = O.unit_cube(3, 4, 2)
+    p2 = O.P2(m)
+    cd = p2.cell_dofs().reshape(-1, 10).astype(np.int64)
+    C = m.cells.astype(np.int64)
+    assert np.array_equal(cd[:, :4], C)
+    pairs = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+    ne = p2.ndofs - m.nv
+    key = np.zeros((ne, 2), dtype=np.int64)
+    for k, (i, j) in enumerate(pairs):
+        e = cd[:, 4 + k] - m.nv
+        key[e, 0] = np.minimum(C[:, i], C[:, j])
+        key[e, 1] = np.maximum(C[:, i], C[:, j])
+    around = np.bincount((cd[:, 4:] - m.nv).ravel(), minlength=ne)
+    order = sorted(range(ne), key=lambda e: (-around[e], key[e, 0], key[e, 1]))
+    assert order == list(range(ne))
+    assert len(np.unique(key, axis=0)) == ne
+    X = p2.dof_coords()
+    V = m.vertices
+    assert np.array_equal(X[m.nv:], (V[key[:, 0]] + V[key[:, 1]]) * 0.5)
+
+
 def test_p2_quadratic_patch():
     """A quadratic u with -Lap u = 6 (u = x^2+y^2+z^2) is reproduced exactly."""
     m = O.unit_cube(3, 3, 3)
