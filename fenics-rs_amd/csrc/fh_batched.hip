@@ -61,6 +61,8 @@ namespace fh {
 constexpr int NR = 4;    // columns stored per row (block size s <= NR)
 constexpr int BT = 256;  // threads per workgroup (4 waves)
 constexpr int TS = 16;   // SELL slices per row tile: each wave takes 4 of them
+constexpr int PU = 8;    // stored entries per step of the product (4: 729 ms, 8: 671 ms of
+                         // product time in the 256-instance mixed solve)
 constexpr int TR = TS * 64;  // rows per tile (each lane accumulates 4 rows before the reduction)
 constexpr int NSYM = 10; // entries of a symmetric 4 x 4 matrix
 
@@ -369,24 +371,25 @@ __global__ __launch_bounds__(BT) void k_bb_prod(SellB<TM> A, const TV* __restric
             const uint32_t* cp = A.cols + (uint64_t)base + lane;
             const TM* vp = A.vals + (uint64_t)base + lane;
             double qv[NR] = {0.0, 0.0, 0.0, 0.0};
-            // four stored entries per step, every load of a step issued before
-            // the arithmetic; SELL padding holds zero values and the row's own
+            // PU stored entries per step (a P1 row of a structured 2D mesh has
+            // 7: one step), every load of a step issued before the
+            // arithmetic; SELL padding holds zero values and the row's own
             // column, so it needs no length test (a step past the width
             // re-reads the last entry with a zero value)
-            for (uint32_t e0 = 0; e0 < wd; e0 += 4) {
-                uint32_t c[4];
-                double v[4];
+            for (uint32_t e0 = 0; e0 < wd; e0 += PU) {
+                uint32_t c[PU];
+                double v[PU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < PU; ++u) {
                     const uint32_t e = min(e0 + u, wd - 1);
                     c[u] = cp[(uint64_t)e * SLICE];
                     v[u] = e0 + u < wd ? (double)vp[(uint64_t)e * SLICE] : 0.0;
                 }
-                double wc[4][NR];
+                double wc[PU][NR];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) ld4(w + (uint64_t)c[u] * NR, wc[u]);
+                for (int u = 0; u < PU; ++u) ld4(w + (uint64_t)c[u] * NR, wc[u]);
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < PU; ++u)
 #pragma unroll
                     for (int k = 0; k < NR; ++k) qv[k] = fma(v[u], wc[u][k], qv[k]);
             }
