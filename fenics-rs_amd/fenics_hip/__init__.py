@@ -332,7 +332,8 @@ class Mesh:
         _check(lib().fh_quad_points(self.h, degree, _p(out)))
         return out
 
-    # P2 tetrahedral dofs (vertices, then edges in lexicographic order)
+    # P2 tetrahedral dofs (vertices, then edges by the number of cells around
+    # them, most first, ties in lexicographic (min, max) order)
     def p2_info(self):
         nd, ne = u64(), u64()
         _check(lib().fh_p2_info(self.h, C.byref(nd), C.byref(ne)))

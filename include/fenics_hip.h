@@ -142,9 +142,10 @@ int fh_assemble_elasticity(fh_mesh* m, double E, double nu, const double* E_cell
                            fh_matrix** out);
 int fh_assemble_elasticity_dev(fh_mesh* m, double E, double nu, const double* E_cell_dev,
                                fh_matrix** out);
-/* P2 tetrahedral Poisson stiffness (new): dofs = vertices, then edges in
- * lexicographic (min,max) order; method 0 = VALU FP64 (exact order of the
- * oracle), 1 = MFMA FP32 element matrices (FP32 accumulate)               */
+/* P2 tetrahedral Poisson stiffness (new): dofs = vertices (their own
+ * numbers), then edges ordered by the number of cells around them, most
+ * first, ties in lexicographic (min,max) order; method 0 = VALU FP64 (exact
+ * order of the oracle), 1 = MFMA FP32 element matrices (FP32 accumulate)  */
 int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out);
 int fh_p2_info(fh_mesh* m, uint64_t* ndofs, uint64_t* nedges);
 int fh_p2_cell_dofs(fh_mesh* m, uint64_t* out /* [nc][10] */);
