@@ -200,6 +200,7 @@ def assembly_once(fh, np, torch, dev, nx, reps):
     cube = fh.unit_cube(64, 64, 64, device=dev)
     runs = [("elasticity", lambda: fh.assemble_elasticity_stiffness_3d(mesh, mat)),
             ("elasticity E_cell", lambda: fh.assemble_elasticity_stiffness_3d(mesh, mat, ec)),
+            ("elasticity mfma", lambda: fh.assemble_elasticity_stiffness_3d(mesh, mat, method=1)),
             ("p2 valu", lambda: fh.assemble_p2_stiffness(cube, 0)),
             ("p2 mfma", lambda: fh.assemble_p2_stiffness(cube, 1))]
     for name, fn in runs:
@@ -447,6 +448,16 @@ def main():
                                 "condensation": (tcd - tb) * 1e3, "solve": (td - tcd) * 1e3,
                                 "note": "host wall time per call; assembly and condensation "
                                         "include the host copies of b"}
+        # the stiffness alone by element path (pattern already built; wall time
+        # around a device sync): FP64 VALU, and FP32 MFMA B^T (vol D) B + row sums
+        out["elasticity_assembly_ms"] = {}
+        for method, name in ((0, "valu_fp64"), (1, "mfma_fp32")):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            K = fh.assemble_elasticity_stiffness_3d(mesh, fh.ElasticMaterial(E, nu), method=method)
+            torch.cuda.synchronize()
+            out["elasticity_assembly_ms"][name] = (time.perf_counter() - t0) * 1e3
+            del K
         xh, xl = torch.empty_like(bt), torch.empty_like(bt)
         rs = Kb.solve_refined_dev(bt, xh, xl, target=1e-14, max_outer=8, inner_tol=1e-6,
                                   max_iter=50000, allow_fail=True)

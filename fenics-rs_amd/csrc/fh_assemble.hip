@@ -377,6 +377,110 @@ __global__ __launch_bounds__(64) void k_asm_elasticity_rows(
     }
 }
 
+// Row gather of stored FP32 elasticity element blocks (the MFMA path,
+// k_elas_elem_mfma): block (a, b) of cell c is components 0-7 at
+// Ke8[((4c + a) 4 + b) 8 ..] and component 8 at Ke9[(4c + a) 4 + b].  The
+// row walk, the LDS sums (elas_acc_index, in floats) and the write-out are
+// those of k_asm_elasticity_rows: lane q of a row's four adds the block of
+// cell vertex b = q, in ascending cell order, and the slice's values are
+// written once, as FP32 and (vals64 given) as their FP64 widening.
+constexpr int ELAS_GRING = 4;  // element blocks in flight per lane
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+struct ElasBlk {
+    f4v_t lo, hi;
+    float v8;
+    uint32_t k;
+};
+template <typename S>
+__device__ __forceinline__ void elas_blk_load(const float* __restrict__ Ke8,
+                                              const float* __restrict__ Ke9,
+                                              const uint32_t* __restrict__ aent,
+                                              const S* __restrict__ slots, uint32_t e, uint32_t q,
+                                              ElasBlk& x) {
+    const uint64_t blk = (uint64_t)aent[e] * 4 + q;  // aent = 4 cell + a
+    const f4v_t* p = reinterpret_cast<const f4v_t*>(Ke8 + blk * 8);
+    x.lo = p[0];
+    x.hi = p[1];
+    x.v8 = Ke9[blk];
+    x.k = slots[(uint64_t)e * 4 + q];
+}
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_gather_elas_rows(
+    const float* __restrict__ Ke8, const float* __restrict__ Ke9,
+    const uint32_t* __restrict__ aptr, const uint32_t* __restrict__ aent,
+    const S* __restrict__ slots, const uint32_t* __restrict__ soff,
+    const uint32_t* __restrict__ rlen, uint32_t nv, uint32_t wp, uint32_t nunits, uint32_t per,
+    float* __restrict__ vals32, double* __restrict__ vals64) {
+    extern __shared__ float accf[];
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    for (uint32_t j = blockIdx.x / 8; j < per; j += gridDim.x / 8) {
+        const uint32_t u = row_unit(blockIdx.x % 8, j);
+        if (u >= nunits) break;
+        const uint32_t sl = u / 4, h = u % 4, lane = threadIdx.x;
+        const uint32_t q = lane & 3, racc = lane >> 2, row = sl * SLICE + h * ELAS_RPW + racc;
+        const uint32_t base = soff[sl], w = (soff[sl + 1] - base) / SLICE;
+        const bool live = row < nv;
+        const uint32_t e0 = live ? aptr[row] : 0, e1 = live ? aptr[row + 1] : 0;
+        const uint32_t wr = lane % ELAS_RPW, wpp = lane / ELAS_RPW;
+        const uint32_t wrow = sl * SLICE + h * ELAS_RPW + wr;
+        const uint32_t wlen = wrow < nv ? rlen[wrow] : 0;
+        for (uint32_t k0 = 0; k0 < w; k0 += wp) {
+            const uint32_t nk = min(wp, w - k0);
+            for (uint32_t i = lane; i < elas_acc_size(nk); i += SLICE) accf[i] = -0.0f;
+            __syncthreads();
+            if (e0 < e1) {
+                const uint32_t last = e1 - 1;
+                ElasBlk x[ELAS_GRING];
+#pragma unroll
+                for (int i = 0; i < ELAS_GRING; ++i)
+                    elas_blk_load(Ke8, Ke9, aent, slots, min(e0 + i, last), q, x[i]);
+                for (uint32_t e = e0; e < e1; e += ELAS_GRING) {
+#pragma unroll
+                    for (int i = 0; i < ELAS_GRING; ++i) {
+                        if (e + i < e1 && x[i].k - k0 < nk) {
+                            float* ab = accf + elas_acc_index(racc, x[i].k - k0, 0);
+                            const float v[9] = {x[i].lo.x, x[i].lo.y, x[i].lo.z, x[i].lo.w,
+                                                x[i].hi.x, x[i].hi.y, x[i].hi.z, x[i].hi.w,
+                                                x[i].v8};
+                            float old[9];
+#pragma unroll
+                            for (int c = 0; c < 9; ++c) old[c] = ab[c * 64];
+#pragma unroll
+                            for (int c = 0; c < 9; ++c) ab[c * 64] = old[c] + v[c];
+                        }
+                        // the entry ELAS_GRING ahead (past the row's last: the last again, unused)
+                        elas_blk_load(Ke8, Ke9, aent, slots, min(e + i + ELAS_GRING, last), q, x[i]);
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t kk = 0; kk < nk; ++kk) {
+                const uint32_t k = k0 + kk;
+                const uint64_t o = ((uint64_t)base + (uint64_t)k * SLICE) * 9;
+                const bool on = k < wlen;
+                const float* ak = accf + elas_acc_index(wr, kk, 0);
+                f2 v;
+                v.x = on ? ak[(2 * wpp) * 64] : 0.0f;
+                v.y = on ? ak[(2 * wpp + 1) * 64] : 0.0f;
+                const uint64_t pi = o + 128 * wpp + 2 * (h * ELAS_RPW + wr);
+                *reinterpret_cast<f2*>(vals32 + pi) = v;
+                const float v8 = on ? ak[8 * 64] : 0.0f;
+                if (wpp == 0) vals32[o + 512 + h * ELAS_RPW + wr] = v8;
+                if (vals64) {
+                    d2 dv;
+                    dv.x = (double)v.x;
+                    dv.y = (double)v.y;
+                    *reinterpret_cast<d2*>(vals64 + pi) = dv;
+                    if (wpp == 0) vals64[o + 512 + h * ELAS_RPW + wr] = (double)v8;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // mass.rs:86-126: M_{(v,al),(b,al)} += density * vol * (1/10 if a == b else
 // 1/20) for every incident cell, the same scalar mass on each component.
 template <typename S>
@@ -1030,25 +1134,47 @@ static P1Tab p1_table(int dim) {
     return t;
 }
 
+// per-cell records of 16 doubles (one 128-byte line): g (12), vol, and
+// lambda, mu with a per-cell E (k_elas_cell_geom)
+void elas_cell_records(fh_mesh* m, double nu, const double* Ecell_dev, double* G, hipStream_t s) {
+    if (!m->nc) return;
+    if (Ecell_dev)
+        hipLaunchKernelGGL((k_elas_cell_geom<true>), dim3(cdiv(m->nc, 256)), dim3(256), 0, s,
+                           m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, G);
+    else
+        hipLaunchKernelGGL((k_elas_cell_geom<false>), dim3(cdiv(m->nc, 256)), dim3(256), 0, s,
+                           m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, G);
+    FH_LAUNCH_CHECK("elas_cell_geom");
+}
+
+// blocks per row held in LDS at once by the elasticity row kernels (48 blocks:
+// 58.8 KB of doubles); wider rows take passes
+static uint32_t elas_pass_blocks(const Pattern& p) { return std::max(1u, std::min(p.max_width, 48u)); }
+
+void elas_gather_f32(fh_mesh* m, const Pattern& p, const float* Ke8, const float* Ke9,
+                     float* vals32, double* vals64, hipStream_t s) {
+    Adjacency& adj = mesh_adjacency(m);
+    const uint32_t wp = elas_pass_blocks(p);
+    const uint32_t nunits = p.nslices * (SLICE / ELAS_RPW);
+    const uint32_t per = row_units_per_xcd(nunits);
+    if (!nunits) return;
+    with_slots(adj, [&](auto* sl) {
+        hipLaunchKernelGGL((k_gather_elas_rows<FH_SLOT_T(sl)>), dim3(rows_grid(m->dev, per)),
+                           dim3(SLICE), (size_t)elas_acc_size(wp) * sizeof(float), s, Ke8, Ke9,
+                           adj.ptr.p, adj.ent.p, sl, p.slice_off.p, p.row_len.p, m->nv, wp,
+                           nunits, per, vals32, vals64);
+    });
+    FH_LAUNCH_CHECK("gather_elas_rows");
+}
+
 void assemble_elasticity_into(fh_mesh* m, double E, double nu, const double* Ecell_dev,
                               fh_matrix* a, hipStream_t s) {
     Adjacency& adj = mesh_adjacency(m);
     const Pattern& p = *a->pat;
-    // per-cell records of 16 doubles (one 128-byte line): g (12), vol, and
-    // lambda, mu with a per-cell E
     const bool ec = Ecell_dev != nullptr;
     DBuf<double> geom(a->dev, (size_t)m->nc * ELAS_GS);
-    if (m->nc) {
-        if (ec)
-            hipLaunchKernelGGL((k_elas_cell_geom<true>), dim3(cdiv(m->nc, 256)), dim3(256), 0,
-                               s, m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, geom.p);
-        else
-            hipLaunchKernelGGL((k_elas_cell_geom<false>), dim3(cdiv(m->nc, 256)), dim3(256), 0,
-                               s, m->verts.p, m->cells.p, m->nc, nu, Ecell_dev, geom.p);
-        FH_LAUNCH_CHECK("elas_cell_geom");
-    }
-    // blocks per row held in LDS at once (48 blocks: 58.8 KB); wider rows take passes
-    const uint32_t wp = std::max(1u, std::min(p.max_width, 48u));
+    elas_cell_records(m, nu, Ecell_dev, geom.p, s);
+    const uint32_t wp = elas_pass_blocks(p);
     const size_t lds = (size_t)elas_acc_size(wp) * sizeof(double);
     const uint32_t nunits = p.nslices * (SLICE / ELAS_RPW);
     const uint32_t per = row_units_per_xcd(nunits);
@@ -1462,6 +1588,35 @@ int fh_assemble_elasticity(fh_mesh* m, double E, double nu, const double* E_cell
         }
         *out = assemble_elasticity(m, E, nu, ec.p);
         FH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fh_assemble_elasticity_method(fh_mesh* m, double E, double nu, const double* E_cell,
+                                  int method, fh_matrix** out) {
+    return guard([&] {
+        FH_CHECK(m && out, "null argument");
+        FH_CHECK(method >= 0 && method <= 2,
+                 "elasticity method must be 0 (VALU FP64), 1 (MFMA FP32) "
+                 "or 2 (both: FP64 values + MFMA FP32 copy)");
+        FH_CHECK(m->dim == 3, "elasticity needs a tetrahedral mesh");
+        FH_CHECK(nu > -1.0 && nu < 0.5,
+                 "Poisson's ratio must be in (-1.0, 0.5) for physical stability");
+        hipStream_t s = stream_of(m->dev);
+        DBuf<double> ec;
+        if (E_cell) {
+            ec.alloc(m->dev, m->nc);
+            h2d(ec.p, E_cell, m->nc, s);
+        }
+        std::unique_ptr<fh_matrix> a;
+        if (method == 1) {
+            a.reset(new_matrix(m->dev, mesh_pattern(m), 3));
+            a->skipped_cells = mesh_degenerate_cells(m);
+        } else {
+            a.reset(assemble_elasticity(m, E, nu, ec.p));
+        }
+        if (method != 0) assemble_elasticity_mfma(m, E, nu, ec.p, a.get(), method == 1, s);
+        FH_HIP(hipStreamSynchronize(s));
+        *out = a.release();
     });
 }
 

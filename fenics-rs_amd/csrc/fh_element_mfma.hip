@@ -18,6 +18,7 @@
 // ascending cell order (no atomics).
 #include <vector>
 
+#include "fh_geom.hpp"
 #include "fh_internal.hpp"
 
 namespace fh {
@@ -163,6 +164,128 @@ void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s) {
     // fill64, in FP64
     if (!a->vals32.p) a->vals32.alloc(m->dev, std::max<uint64_t>(a->pat->sell_entries, 1));
     p2_gather(m, Ke.p, false, a->vals32.p, fill64 ? a->vals.p : nullptr, s);
+    FH_HIP(hipStreamSynchronize(s));
+}
+
+// P1 tetrahedron elasticity element matrices on the MFMA units, FP32 in and
+// FP32 accumulate: K_e = B^T (vol D) B, with B (6 x 12) the Voigt
+// strain-displacement matrix of the cell's four gradients (rows e_xx, e_yy,
+// e_zz, g_yz, g_xz, g_xy) and D (6 x 6) the isotropic stiffness (lambda + 2 mu
+// and lambda on the normal block, mu on the shear diagonal).  Entry
+// ((a, al), (b, be)) is vol (lambda g_a[al] g_b[be] + mu g_a[be] g_b[al] +
+// mu (g_a . g_b) [al == be]), the elasticity.rs:133-136 entry.
+//
+// v_mfma_f32_16x16x1f32 computes four independent 16 x 16 outer products per
+// instruction: block = cell, A = column k of B^T (16 rows (a, al), 12 used),
+// B = row k of vol D B; K = 6 gives six instructions per four cells.  Lane l
+// of a group owns cell l / 16 and index i = l % 16 of both operands, so it
+// builds its B and vol D B entries from its own vertex gradient g_{i / 3}
+// alone.  The inputs are the FP64 per-cell records of k_elas_cell_geom
+// (gradients, vol, per-cell lambda and mu; vol < 0 marks a cell the
+// reference skips, whose element matrix is zero), rounded to FP32.
+// Result layout (16 registers): register r holds block r / 4, row
+// 4 (l / 16) + r % 4, column l % 16.  The blocks go to Ke8 [nc][4][4][8]
+// (components 0-7 of block (a, b)) and Ke9 [nc][4][4] (component 8),
+// through LDS, as contiguous 16-byte stores.
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <bool EC>
+__global__ __launch_bounds__(256) void k_elas_elem_mfma(const double* __restrict__ G, uint32_t nc,
+                                                        float lam0, float mu0,
+                                                        float* __restrict__ Ke8,
+                                                        float* __restrict__ Ke9) {
+    // a wave's four element matrices, staged in the output order (Ke8 part,
+    // then Ke9 part) so that they leave as contiguous 16-byte stores
+    __shared__ __align__(16) float st[4][576];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t i = lane & 15, al = i % 3, ia = i / 3;
+    const uint32_t ngroups = (nc + 3) / 4;
+    float* sw = st[wave];
+    // block-uniform trip count (the barriers below); groups past the end are
+    // all-masked
+    for (uint32_t gbase = blockIdx.x * 4; gbase < ngroups; gbase += gridDim.x * 4) {
+        const uint32_t c4 = (gbase + wave) * 4, c = c4 + lane / 16;
+        float g[3] = {0.f, 0.f, 0.f}, vol = 0.f, lam = lam0, mu = mu0;
+        if (c < nc) {
+            const double* rec = G + (uint64_t)c * 16;
+            const double v = rec[3];
+            if (v > 0.0 && i < 12) {
+                vol = (float)v;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) g[d] = (float)rec[4 * ia + d];
+                if (EC) {
+                    lam = (float)rec[7];
+                    mu = (float)rec[11];
+                }
+            }
+        }
+        // column (a, al) of B: its entries in rows e_xx .. g_xy
+        float Bk[6];
+        Bk[0] = al == 0 ? g[0] : 0.f;
+        Bk[1] = al == 1 ? g[1] : 0.f;
+        Bk[2] = al == 2 ? g[2] : 0.f;
+        Bk[3] = al == 1 ? g[2] : (al == 2 ? g[1] : 0.f);
+        Bk[4] = al == 0 ? g[2] : (al == 2 ? g[0] : 0.f);
+        Bk[5] = al == 0 ? g[1] : (al == 1 ? g[0] : 0.f);
+        const float tr = Bk[0] + Bk[1] + Bk[2];
+        const float vl = vol * lam, vm = vol * mu;
+        float DBk[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) DBk[k] = vl * tr + 2.f * vm * Bk[k];
+#pragma unroll
+        for (int k = 3; k < 6; ++k) DBk[k] = vm * Bk[k];
+        v16f acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x1f32(Bk[k], DBk[k], acc, 0, 0, 0);
+        const uint32_t col = lane & 15, b = col / 3, be = col % 3;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t cl = r / 4, row = 4 * (lane / 16) + r % 4;
+            if (row < 12 && col < 12) {
+                const uint32_t e = (row % 3) * 3 + be, blk = (row / 3) * 4 + b;
+                if (e < 8)
+                    sw[cl * 128 + blk * 8 + e] = acc[r];
+                else
+                    sw[512 + cl * 16 + blk] = acc[r];
+            }
+        }
+        __syncthreads();
+        // cells c4 .. c4 + 3: Ke8 [c4 * 128, + 512) as 128 float4, Ke9 [c4 * 16, + 64)
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t f = lane + 64 * h;  // float4 index: cell c4 + f / 32
+            if (c4 + f / 32 < nc)
+                reinterpret_cast<f4*>(Ke8 + (uint64_t)c4 * 128)[f] =
+                    reinterpret_cast<const f4*>(sw)[f];
+        }
+        if (c4 + lane / 16 < nc) Ke9[(uint64_t)c4 * 16 + lane] = sw[512 + lane];
+        __syncthreads();
+    }
+}
+
+void assemble_elasticity_mfma(fh_mesh* m, double E, double nu, const double* Ecell_dev,
+                              fh_matrix* a, bool fill64, hipStream_t s) {
+    const size_t nc = m->nc;
+    DBuf<double> geom(m->dev, std::max<size_t>(nc * 16, 1));
+    elas_cell_records(m, nu, Ecell_dev, geom.p, s);
+    DBuf<float> Ke8(m->dev, std::max<size_t>(nc * 128, 1)), Ke9(m->dev, std::max<size_t>(nc * 16, 1));
+    if (nc) {
+        const uint32_t ngroups = (uint32_t)((nc + 3) / 4);
+        const unsigned g = std::max(1u, std::min<unsigned>(cdiv(ngroups, 4), 8192));
+        const float lam = (float)fhg::lame_lambda(E, nu), mu = (float)fhg::lame_mu(E, nu);
+        if (Ecell_dev)
+            hipLaunchKernelGGL((k_elas_elem_mfma<true>), dim3(g), dim3(256), 0, s, geom.p,
+                               (uint32_t)nc, lam, mu, Ke8.p, Ke9.p);
+        else
+            hipLaunchKernelGGL((k_elas_elem_mfma<false>), dim3(g), dim3(256), 0, s, geom.p,
+                               (uint32_t)nc, lam, mu, Ke8.p, Ke9.p);
+        FH_LAUNCH_CHECK("elas_elem_mfma");
+    }
+    if (!a->vals32.p) a->vals32.alloc(m->dev, std::max<uint64_t>(sell_value_count(a), 1));
+    elas_gather_f32(m, *a->pat, Ke8.p, Ke9.p, a->vals32.p, fill64 ? a->vals.p : nullptr, s);
     FH_HIP(hipStreamSynchronize(s));
 }
 

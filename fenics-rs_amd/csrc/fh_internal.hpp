@@ -175,6 +175,17 @@ struct HostCsr {
 };
 HostCsr export_scalar_csr(const fh_matrix* a);
 void assemble_p2_mfma(fh_mesh* m, fh_matrix* a, bool fill64, hipStream_t s);
+// P1 elasticity on the MFMA units (fh_element_mfma.hip): FP32 element
+// matrices, summed per row into a->vals32 (and, with fill64, their FP64
+// widening into a->vals)
+void assemble_elasticity_mfma(fh_mesh* m, double E, double nu, const double* Ecell_dev,
+                              fh_matrix* a, bool fill64, hipStream_t s);
+// per-cell gradient records [nc][16] (fh_assemble.hip k_elas_cell_geom)
+void elas_cell_records(fh_mesh* m, double nu, const double* Ecell_dev, double* G, hipStream_t s);
+// sum FP32 elasticity element blocks (Ke8 [nc][4][4][8], Ke9 [nc][4][4]) into
+// the SELL values of pattern p (vals64: also as doubles, when not null)
+void elas_gather_f32(fh_mesh* m, const Pattern& p, const float* Ke8, const float* Ke9,
+                     float* vals32, double* vals64, hipStream_t s);
 // sum P2 element matrices Ke [nc][10][10] (FP64, or FP32 when !f64) into SELL
 // values; FP32 sums are also written as doubles to vals64 when it is not null
 void p2_gather(fh_mesh* m, const void* Ke, bool f64, void* vals, double* vals64, hipStream_t s);

@@ -123,6 +123,7 @@ def lib():
         "fh_assemble_mass": (C.c_int, [vp, PP]),
         "fh_assemble_elasticity": (C.c_int, [vp, dbl, dbl, vp, PP]),
         "fh_assemble_elasticity_dev": (C.c_int, [vp, dbl, dbl, vp, PP]),
+        "fh_assemble_elasticity_method": (C.c_int, [vp, dbl, dbl, vp, C.c_int, PP]),
         "fh_assemble_p2_stiffness": (C.c_int, [vp, C.c_int, PP]),
         "fh_p2_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "fh_p2_cell_dofs": (C.c_int, [vp, vp]),
@@ -612,11 +613,18 @@ class ElasticMaterial:
 
 
 def assemble_elasticity_stiffness_3d(mesh: Mesh, material: ElasticMaterial,
-                                     E_cell=None) -> SparseMatrix:
-    """elasticity.rs:73-149 (E_cell: optional per-cell Young's moduli)."""
+                                     E_cell=None, method: int = 0) -> SparseMatrix:
+    """elasticity.rs:73-149 (E_cell: optional per-cell Young's moduli).
+
+    method 0: FP64 VALU (the reference's values, bitwise); 1: FP32 element
+    matrices B^T (vol D) B on the MFMA units, summed per row in FP32; 2: the
+    FP64 values of method 0 with the MFMA FP32 values as the FP32 copy."""
     ec = None if E_cell is None else _f64(E_cell)
-    return _new_matrix(lib().fh_assemble_elasticity, mesh.h, material.youngs_modulus,
-                       material.poissons_ratio, _p(ec))
+    if method == 0:
+        return _new_matrix(lib().fh_assemble_elasticity, mesh.h, material.youngs_modulus,
+                           material.poissons_ratio, _p(ec))
+    return _new_matrix(lib().fh_assemble_elasticity_method, mesh.h, material.youngs_modulus,
+                       material.poissons_ratio, _p(ec), int(method))
 
 
 def assemble_elasticity_body_force_3d(mesh: Mesh, f_body) -> np.ndarray:

@@ -142,6 +142,14 @@ int fh_assemble_elasticity(fh_mesh* m, double E, double nu, const double* E_cell
                            fh_matrix** out);
 int fh_assemble_elasticity_dev(fh_mesh* m, double E, double nu, const double* E_cell_dev,
                                fh_matrix** out);
+/* elasticity.rs:73-149 as above, by method: 0 = VALU FP64 (the values of
+ * fh_assemble_elasticity, bitwise the reference's order); 1 = MFMA FP32:
+ * element matrices B^T (vol D) B on the matrix cores (B the 6x12 Voigt
+ * strain-displacement matrix, D the 6x6 isotropic stiffness), summed per row
+ * in FP32, the FP64 values their widening; 2 = both: the FP64 values of
+ * method 0 and the MFMA FP32 values as the matrix's FP32 copy             */
+int fh_assemble_elasticity_method(fh_mesh* m, double E, double nu, const double* E_cell,
+                                  int method, fh_matrix** out);
 /* P2 tetrahedral Poisson stiffness (new): dofs = vertices (their own
  * numbers), then edges ordered by the number of cells around them, most
  * first, ties in lexicographic (min,max) order; method 0 = VALU FP64 (exact

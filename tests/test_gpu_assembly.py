@@ -101,6 +101,52 @@ def test_elasticity_bitwise(E):
                           bits(O.assemble_body_force(om, f)))
 
 
+@pytest.mark.parametrize("E", [210e9, "cells"])
+def test_elasticity_mfma_matches_fp64(E):
+    """FP32 element matrices B^T (vol D) B on the MFMA units (method 1),
+    summed per row in FP32, against the FP64 assembly: the same pattern, every
+    value within 2e-6 of the largest entry (FP32 rounding of the gradients,
+    vol, lambda and mu, of each entry's six-term MFMA sum, and of up to 24
+    additions per stored value; one cell's contribution is about max|K| / 24,
+    so a dropped, doubled or misplaced cell fails it), rigid translations in
+    the kernel to FP32 rounding.  Method 2 keeps the FP64 values bitwise the
+    oracle's and carries the MFMA values as its FP32 copy."""
+    dims = (1.0, 0.3, 0.2, 9, 4, 3)
+    gm, om = fh.box_beam(*dims), O.box_beam(*dims)
+    mat = fh.ElasticMaterial(2e11, 0.3)
+    ec = None if E != "cells" else np.random.default_rng(5).uniform(1e10, 3e11, om.nc)
+    c1, r1, v1 = fh.assemble_elasticity_stiffness_3d(gm, mat, ec).export_csc()
+    K32 = fh.assemble_elasticity_stiffness_3d(gm, mat, ec, method=1)
+    c2, r2, v2 = K32.export_csc()
+    assert np.array_equal(c1, c2) and np.array_equal(r1, r2)
+    assert np.abs(v2 - v1).max() <= 2e-6 * np.abs(v1).max()
+    assert np.array_equal(v2, v2.astype(np.float32).astype(np.float64))
+    for a in range(3):
+        t = np.zeros((gm.nv, 3))
+        t[:, a] = 1.0
+        assert np.abs(K32.spmv(t.reshape(-1))).max() <= 1e-5 * np.abs(v1).max()
+    K2 = fh.assemble_elasticity_stiffness_3d(gm, mat, ec, method=2)
+    same_csc(K2, O.assemble_elasticity(om, ec if ec is not None else 2e11, 0.3))
+
+
+def test_elasticity_mfma_copy_drives_the_mixed_solve():
+    """Method 2 (FP64 values, MFMA FP32 copy): the mixed solve of a clamped
+    cantilever reaches the FP64 solve's true residual and solution."""
+    gm = fh.box_beam(2.0, 0.4, 0.4, 20, 4, 4)
+    mat = fh.ElasticMaterial(210e9, 0.3)
+    b = fh.assemble_elasticity_body_force_3d(gm, [0, 0, -7.85e4])
+    bc = fh.clamp_dofs_x0(gm.vertices)
+    out = {}
+    for method in (0, 2):
+        K = fh.assemble_elasticity_stiffness_3d(gm, mat, method=method)
+        Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
+        prec = fh.FP64_TRUE if method == 0 else fh.MIXED
+        out[method] = Kb.solve(bb, tol=1e-10, max_iter=20000, precision=prec)
+    (x0, s0), (x2, s2) = out[0], out[2]
+    assert s2.true_rel_residual <= max(1e-10, 2.5 * s0.true_rel_residual)
+    assert np.linalg.norm(x2 - x0) / np.linalg.norm(x0) < 1e-8
+
+
 def test_dirichlet_scalar_bitwise():
     gm, om = fh.unit_square(10, 8), O.unit_square(10, 8)
     b = np.random.default_rng(2).standard_normal(om.nv)
@@ -183,6 +229,22 @@ def test_c1_full_size_counts_and_invariants():
     assert abs(b.sum() - (-7.85e4 * 40.0)) < 1e-9 * 7.85e4 * 40.0
     Kb, bb = fh.apply_vector_dirichlet_bc(K, b, bc)
     assert Kb.nnz() == c["nnz_bc"]
+
+
+def test_c1_elasticity_mfma_matches_fp64():
+    """The FP32 MFMA elasticity assembly at full C1 size: the pattern of the
+    FP64 assembly, every value within 2e-6 of the largest entry (one cell's
+    contribution is about max|K| / 24, so a dropped or doubled cell fails it),
+    and every value an FP32 number."""
+    gm = fh.box_beam(10.0, 2.0, 2.0, 200, 40, 40)
+    mat = fh.ElasticMaterial(210e9, 0.3)
+    c1, r1, v1 = fh.assemble_elasticity_stiffness_3d(gm, mat).export_csc()
+    c2, r2, v2 = fh.assemble_elasticity_stiffness_3d(gm, mat, method=1).export_csc()
+    assert np.array_equal(c1, c2) and np.array_equal(r1, r2)
+    err = np.abs(v2 - v1).max() / np.abs(v1).max()
+    print(f"C1 MFMA elasticity: max |K32 - K64| / max |K64| = {err:.3e}")
+    assert err <= 2e-6
+    assert np.array_equal(v2, v2.astype(np.float32).astype(np.float64))
 
 
 def test_c2_full_size_p2_counts():
@@ -282,6 +344,13 @@ def test_high_valence_vertex_bitwise():
     k = np.bincount(gm.p2_cell_dofs().ravel().astype(np.int64)).max()
     assert k == len(C)
     assert np.abs(v32 - v64).max() <= (k + 16) * 2.0 ** -24 * np.abs(v64).max()
+    # the FP32 MFMA elasticity path through the multi-pass row gather (the
+    # centre's row of 181 blocks is summed in four passes of 48), under the
+    # same recursive-summation bound with the centre's 356 cells
+    mat = fh.ElasticMaterial(7e10, 0.3)
+    _, _, e64 = fh.assemble_elasticity_stiffness_3d(gm, mat).export_csc()
+    _, _, e32 = fh.assemble_elasticity_stiffness_3d(gm, mat, method=1).export_csc()
+    assert np.abs(e32 - e64).max() <= (k + 16) * 2.0 ** -24 * np.abs(e64).max()
 
 
 @pytest.mark.parametrize("dim", [2, 3])
