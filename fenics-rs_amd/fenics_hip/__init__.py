@@ -334,7 +334,7 @@ class Mesh:
         return out
 
     # P2 tetrahedral dofs (vertices, then edges by the number of cells around
-    # them, most first, ties in lexicographic (min, max) order)
+    # them, most first, then by max - min, then by min)
     def p2_info(self):
         nd, ne = u64(), u64()
         _check(lib().fh_p2_info(self.h, C.byref(nd), C.byref(ne)))

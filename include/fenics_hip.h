@@ -152,7 +152,7 @@ int fh_assemble_elasticity_method(fh_mesh* m, double E, double nu, const double*
                                   int method, fh_matrix** out);
 /* P2 tetrahedral Poisson stiffness (new): dofs = vertices (their own
  * numbers), then edges ordered by the number of cells around them, most
- * first, ties in lexicographic (min,max) order; method 0 = VALU FP64 (exact
+ * first, then by max - min, then by min; method 0 = VALU FP64 (exact
  * order of the oracle), 1 = MFMA FP32 element matrices (FP32 accumulate)  */
 int fh_assemble_p2_stiffness(fh_mesh* m, int method, fh_matrix** out);
 int fh_p2_info(fh_mesh* m, uint64_t* ndofs, uint64_t* nedges);

@@ -6,8 +6,9 @@
 // and the P2 tetrahedral dofmap / stiffness that the reference lacks:
 // vertices first, then edges (the vertices-then-edges convention of
 // mass.rs:132-135), the edges ordered by the number of cells around them,
-// most first (counts above 255 count as 255), ties in lexicographic
-// (min, max) vertex order; per-cell dofs in
+// most first (counts above 255 count as 255), then by the vertex difference
+// max - min (edges of one direction of a structured mesh), then by the
+// smaller vertex; per-cell dofs in
 // P2Tetrahedron order (lagrange.rs:284-297: corners, then edges 01, 12, 02,
 // 03, 13, 23); K_e = sum_q (w_q |det J|) (g_i^q . g_j^q) with the degree-2
 // tetrahedron rule (quadrature.rs:157-184), exact for P2 x P2 gradients.
@@ -321,11 +322,16 @@ or_p2* or_p2_build(uint64_t nv, const double* verts, uint64_t nc, const uint64_t
         p->edges.swap(uniq);
     }
     p->ne = p->edges.size();
-    // edge dof numbers: most cells around first, ties in lexicographic order
+    // edge dof numbers: most cells around first, then by max - min, then by
+    // min (the edges are in lexicographic order, so the stable sort keeps
+    // ascending min among equal keys)
     std::vector<uint64_t> order(p->ne);
     for (uint64_t e = 0; e < p->ne; ++e) order[e] = e;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint64_t a, uint64_t b) { return around[a] > around[b]; });
+    auto span = [&](uint64_t e) { return p->edges[e][1] - p->edges[e][0]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+        if (around[a] != around[b]) return around[a] > around[b];
+        return span(a) < span(b);
+    });
     p->erank.assign(p->ne, 0);
     for (uint64_t r = 0; r < p->ne; ++r) p->erank[order[r]] = r;
     p->cell_dofs.resize(10 * nc);

@@ -385,8 +385,8 @@ def test_p2_dofmap_counts_unit_cube():
 
 def test_p2_edge_numbering_convention():
     """P2 dofs: vertices keep their numbers; edge dofs follow ordered by the
-    number of cells around the edge (most first), ties in lexicographic
-    (min, max) vertex order; every edge's midpoint is its dof coordinate."""
+    number of cells around the edge (most first), then by max - min, then by
+    min; every edge's midpoint is its dof coordinate."""
     m = O.unit_cube(3, 4, 2)
     p2 = O.P2(m)
     cd = p2.cell_dofs().reshape(-1, 10).astype(np.int64)
@@ -400,7 +400,7 @@ def test_p2_edge_numbering_convention():
         key[e, 0] = np.minimum(C[:, i], C[:, j])
         key[e, 1] = np.maximum(C[:, i], C[:, j])
     around = np.bincount((cd[:, 4:] - m.nv).ravel(), minlength=ne)
-    order = sorted(range(ne), key=lambda e: (-around[e], key[e, 0], key[e, 1]))
+    order = sorted(range(ne), key=lambda e: (-around[e], key[e, 1] - key[e, 0], key[e, 0]))
     assert order == list(range(ne))
     assert len(np.unique(key, axis=0)) == ne
     X = p2.dof_coords()
