@@ -309,8 +309,9 @@ __device__ __forceinline__ double precond_row(const double* __restrict__ dinv, u
 // EXACT: t in double-double (true-residual modes: initial residual and
 // replacement) plus the rounding-floor partials; otherwise FP64 in the
 // reference's product order (ORD).  z goes to the z slots of the (z, p) pair
-// vector, p is left as it is.  Every lane writes r and z (zeros past the last
-// row), so the padded vector entries stay zero.
+// vector, p is left as it is.  Every lane writes r (unless r is null: the
+// update then keeps no r) and z (zeros past the last row), so the padded
+// vector entries stay zero.
 template <int BS, bool ORD, bool EXACT, int PRE>
 __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restrict__ b,
                                                   const double* __restrict__ x,
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(256) void k_residual(CgMat A, const double* __restr
 #pragma unroll
         for (int a = 0; a < BS; ++a) {
             if (!live) rv[a] = 0.0;
-            r[((uint64_t)s * BS + a) * SLICE + lane] = rv[a];
+            if (r) r[((uint64_t)s * BS + a) * SLICE + lane] = rv[a];
             tt += rv[a] * rv[a];
             bsq += bi[a] * bi[a];
         }
@@ -887,6 +888,10 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
 // PRE 1: 3x3 block Jacobi (gpu.rs bsr3_apply_dinv), one thread per block row.
 // p_new = z + beta p_old is rebuilt from the (z, p) pairs exactly as the spmv
 // kernel formed it; the pairs are overwritten with (z_new, p_new).
+// r null (PRE 0, a D^{-1} with no zero entry): r is not stored; its entries
+// come back from the pairs as z_i / D^{-1}_i (within two roundings of the r
+// that z was formed from), which saves 16 of the 80 bytes per unknown this
+// kernel streams.
 template <bool ORD, int PRE, bool EARLY>
 __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __restrict__ x,
                                                    double* __restrict__ r,
@@ -906,12 +911,13 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
     // the state is read before the reduction.  One element per thread covers
     // that prologue; more only add to what a no-op launch streams.
     constexpr bool PF = EARLY && PRE == 0;
+    const bool zr = r == nullptr;
     pair_t f_zp = {0.0, 0.0};
     double f_x = 0.0, f_r = 0.0, f_ap = 0.0, f_d = 0.0;
     if (PF && i0 < npad) {
         f_zp = zpv[i0];
         f_x = x[i0];
-        f_r = r[i0];
+        if (!zr) f_r = r[i0];
         f_ap = Ap[i0];
         f_d = dinv[i0];
     }
@@ -944,10 +950,11 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
         auto step = [&](uint64_t i, pair_t o, double xo, double ro, double ap, double di) {
             const double pi = madd(o.x, beta, o.y, ORD);
             const double xi = ORD ? xo + al * pi : fma(al, pi, xo);
-            const double ri = ORD ? ro - al * ap : fma(-al, ap, ro);
+            const double rp = zr ? o.x / di : ro;
+            const double ri = ORD ? rp - al * ap : fma(-al, ap, rp);
             const double zi = ri * di;
             x[i] = xi;
-            r[i] = ri;
+            if (!zr) r[i] = ri;
             pair_t o2;
             o2.x = zi;
             o2.y = pi;
@@ -960,7 +967,7 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
             if (i < npad) step(i, f_zp, f_x, f_r, f_ap, f_d);
             i += stride;
         }
-        for (; i < npad; i += stride) step(i, zpv[i], x[i], r[i], Ap[i], dinv[i]);
+        for (; i < npad; i += stride) step(i, zpv[i], x[i], zr ? 0.0 : r[i], Ap[i], dinv[i]);
     } else {
         const uint32_t nn = npad / 3;
         for (uint32_t v = blockIdx.x * UPD_T + threadIdx.x; v < nn; v += stride) {
@@ -1481,11 +1488,15 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // workspace and graphs: kept on the matrix per kernel configuration (the
     // graphs reference only the matrix and the workspace), so a repeated solve
     // of this operator allocates and captures nothing
+    // scalar Jacobi with the library's own D^{-1} (never zero): no stored r,
+    // its entries are recovered from z (k_cg_update); the reference's ordered
+    // arithmetic and a caller's D^{-1} keep r
+    const bool keep_r = PRE != 0 || ORD || args.inv_diag_override;
     CgCache* cc = nullptr;
     if (!noalloc) {
         const uint64_t key = (uint64_t)BS | (uint64_t)ORD << 4 | (uint64_t)DIFF << 5 |
                              (uint64_t)PRE << 6 | (uint64_t)diff64 << 8 | (uint64_t)aprec << 9 |
-                             (uint64_t)M << 16 | (uint64_t)gc << 40;
+                             (uint64_t)keep_r << 10 | (uint64_t)M << 16 | (uint64_t)gc << 40;
         std::unique_ptr<CgCache>& slot = am->cg_cache[key];
         if (!slot) {
             slot = std::make_unique<CgCache>();
@@ -1497,6 +1508,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     CgWork w{};
     cg_work(a, PRE, DIFF, gc, ws, &w);
     double* const x = w.xl;
+    double* const rvec = keep_r ? w.r : nullptr;
     if (PRE == 1) to_li<9>(args.block_dinv, w.dinv, nrows, nsl, 0.0, s);
     else if (args.inv_diag_override) to_li<BS>(args.inv_diag_override, w.dinv, nrows, nsl, 1.0, s);
     else {
@@ -1530,7 +1542,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // r0 = b - A x0 (FP64; double-double in the true-residual modes),
     // z0 = D^{-1} r0, partials (r.r, r.z), b.b
     hipLaunchKernelGGL((k_residual<BS, ORD, DIFF, PRE>), dim3(gs), dim3(256), 0, s, A, w.bl, x,
-                       w.dinv, w.r, w.zp, w.redB, w.bb, w.fl);
+                       w.dinv, rvec, w.zp, w.redB, w.bb, w.fl);
     FH_LAUNCH_CHECK("residual");
     // the next spmv follows k_residual (gs partials) rather than an update (gu)
     bool after_res = true;
@@ -1574,7 +1586,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             }
 #undef FH_SPMV
             hipLaunchKernelGGL((k_cg_update<ORD, PRE, BS == 3 && !ORD>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
-                               w.r, w.Ap, w.zp, w.dinv, w.st, w.redA, (int)gc, w.redB);
+                               rvec, w.Ap, w.zp, w.dinv, w.st, w.redA, (int)gc, w.redB);
         }
     };
     uint64_t launched = 0;
@@ -1630,7 +1642,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             d2d((hs->n_rep & 1) ? w.xs1 : w.xs0, x, npad, s);
             // residual replacement: r = b - A x (double-double evaluation)
             hipLaunchKernelGGL((k_residual<BS, false, true, PRE>), dim3(gs), dim3(256), 0, s, A,
-                               w.bl, x, w.dinv, w.r, w.zp, w.redB, (double*)nullptr, w.fl);
+                               w.bl, x, w.dinv, rvec, w.zp, w.redB, (double*)nullptr, w.fl);
             hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, s, w.st);
             FH_LAUNCH_CHECK("replacement");
             after_res = true;
