@@ -129,6 +129,32 @@ def test_elasticity_mfma_matches_fp64(E):
     same_csc(K2, O.assemble_elasticity(om, ec if ec is not None else 2e11, 0.3))
 
 
+def test_elasticity_mfma_one_cell_and_errors():
+    """One tetrahedron: a single MFMA group with three of its four cells
+    masked and a one-slice gather, against the oracle's FP64 values; a
+    skipped (degenerate) cell adds nothing; unknown methods and a Poisson
+    ratio outside (-1, 0.5) are rejected, as elasticity.rs:14-48 rejects them."""
+    V = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    C = np.array([[0, 1, 2, 3]], dtype=np.uint64)
+    mat = fh.ElasticMaterial(2e11, 0.3)
+    gm, om = fh.Mesh.from_arrays(V, C), O.Mesh(V, C)
+    _, _, v64 = O.assemble_elasticity(om, 2e11, 0.3).export()
+    _, _, v32 = fh.assemble_elasticity_stiffness_3d(gm, mat, method=1).export_csc()
+    assert np.abs(v32 - v64).max() <= 2e-6 * np.abs(v64).max()
+    # the same cell plus a flat copy of it (det J = 0): the values do not change
+    C2 = np.array([[0, 1, 2, 3], [0, 1, 2, 0]], dtype=np.uint64)
+    g2 = fh.Mesh.from_arrays(V, C2)
+    K2 = fh.assemble_elasticity_stiffness_3d(g2, mat, method=1)
+    assert K2.skipped_cells() == 1
+    _, _, w32 = K2.export_csc()
+    assert np.array_equal(w32, v32)
+    for bad in (-1, 3):
+        with pytest.raises(fh.FenicsError, match="method"):
+            fh.assemble_elasticity_stiffness_3d(gm, mat, method=bad)
+    with pytest.raises(fh.FenicsError, match="Poisson"):
+        fh._new_matrix(fh.lib().fh_assemble_elasticity_method, gm.h, 2e11, 0.5, None, 1)
+
+
 def test_elasticity_mfma_copy_drives_the_mixed_solve():
     """Method 2 (FP64 values, MFMA FP32 copy): the mixed solve of a clamped
     cantilever reaches the FP64 solve's true residual and solution."""

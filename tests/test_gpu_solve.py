@@ -261,6 +261,27 @@ def test_csr_cg_vs_cpu():  # gpu.rs:1609-1663
     assert r3.iterations <= 1
 
 
+def test_solve_without_stored_residual_matches_kept_residual():
+    """Scalar Jacobi with the library's own inverse diagonal keeps no residual
+    vector: the update recovers r_i as z_i / D^-1_i from the (z, p) pairs.  A
+    caller-supplied inverse diagonal (the CSR entry point) keeps r.  On the
+    same system, with the same stopping rule, the two take the same number of
+    iterations up to 2, reach the same solution to 1e-8 and true residuals
+    within 2x of tol; a residual recovered wrongly (off by a factor, or
+    stale) fails both."""
+    A = laplace7(24)
+    b = np.random.default_rng(21).uniform(-1.0, 1.0, A.n)
+    rp, ci, va, inv = A.to_csr()
+    kept = fh.solve_csr_cg_gpu(rp, ci, va, inv, b, None, 1e-10, 5000)
+    cp, ri, vv = A.export()
+    zr = fh.solve_cg_gpu_or_cpu(cp, ri, vv, b, None, 1e-10, 5000)
+    assert abs(kept.iterations - zr.iterations) <= 2
+    assert np.linalg.norm(kept.solution - zr.solution) <= 1e-8 * np.linalg.norm(kept.solution)
+    S = A.to_scipy()
+    for r in (kept, zr):
+        assert np.linalg.norm(b - S @ r.solution) <= 2e-10 * np.linalg.norm(b)
+
+
 def bsr3_synthetic(nx=15):
     """gpu.rs:1671-1741: synthetic SPD 3x3 block-CSR on a 15^3 grid."""
     n_nodes = nx ** 3
