@@ -377,6 +377,7 @@ static void order_edges(uint32_t* cd, uint32_t nc, const Pattern& vp, const uint
                         uint32_t nv, uint32_t ne, int dev, hipStream_t s) {
     if (!ne) return;
     FH_CHECK(nv < (1u << 28), "P2 edge numbering needs fewer than 2^28 vertices");
+    FH_CHECK(ne < (1u << 31), "P2 edge numbering needs fewer than 2^31 edges");
     DBuf<uint32_t> cnt(dev, ne), pos(dev, ne), spos(dev, ne), rank(dev, ne);
     DBuf<uint64_t> key(dev, ne), skey(dev, ne);
     fill_u32(cnt.p, ne, 0u, s);
