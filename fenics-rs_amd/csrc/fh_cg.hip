@@ -95,6 +95,8 @@ struct CgState {
     uint32_t stalled;   // FP32-operator cycle failed to reduce the true residual
     double rz[2];
     double beta[2];     // beta of iteration j, by parity (the update rebuilds p with it)
+    double alpha[2];    // alpha of iteration j, by parity (the deferred x update reads it)
+    uint64_t x_base;    // last iteration at which x is exact (deferred x updates; k_x_base)
     uint32_t low[2];    // operator precision of iteration j, by parity
     double bnorm;
     double res;
@@ -892,6 +894,12 @@ __global__ __launch_bounds__(CG_T) void k_cg_spmv(CgMat A, XcdMap xm, const doub
 // come back from the pairs as z_i / D^{-1}_i (within two roundings of the r
 // that z was formed from), which saves 16 of the 80 bytes per unknown this
 // kernel streams.
+// lazy (PRE 0): x is updated every second iteration.  An update an odd number
+// of iterations after x_base leaves x as it is; the next one, which still
+// finds p_{j-1} in the pairs, adds alpha_{j-1} p_{j-1} and then alpha_j p_j:
+// the same two FMAs in the same order as updating every iteration, so x is
+// bitwise the same, for half of the x traffic.  k_x_flush completes x before
+// anything reads it (a replacement, the end of the solve).
 template <bool ORD, int PRE, bool EARLY>
 __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __restrict__ x,
                                                    double* __restrict__ r,
@@ -899,7 +907,7 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
                                                    double* __restrict__ zp,
                                                    const double* __restrict__ dinv, CgState* st,
                                                    const double* __restrict__ red_in, int n_in,
-                                                   double2* __restrict__ red_out) {
+                                                   double2* __restrict__ red_out, int lazy) {
     typedef double pair_t __attribute__((ext_vector_type(2)));
     __shared__ double sh[UPD_T / 64];
     pair_t* zpv = reinterpret_cast<pair_t*>(zp);
@@ -944,16 +952,22 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
     }
     const double al = st->rz[j & 1] / pap;
     const double beta = st->beta[j & 1];
+    // deferred x: skip (x then lags by alpha_j p_j), or catch up the lag
+    const bool skip_x = lazy && ((j - st->x_base) & 1);
+    const bool two = lazy && !skip_x;
+    const double al_prev = two ? st->alpha[(j - 1) & 1] : 0.0;
+    if (lead) st->alpha[j & 1] = al;
     double rr = 0.0, rz = 0.0;
     if (PRE == 0) {
         // cg.rs:112-115 with p of cg.rs:133-135, z of cg.rs:124-126
         auto step = [&](uint64_t i, pair_t o, double xo, double ro, double ap, double di) {
             const double pi = madd(o.x, beta, o.y, ORD);
-            const double xi = ORD ? xo + al * pi : fma(al, pi, xo);
+            const double xl = two ? fma(al_prev, o.y, xo) : xo;
+            const double xi = ORD ? xo + al * pi : fma(al, pi, xl);
             const double rp = zr ? o.x / di : ro;
             const double ri = ORD ? rp - al * ap : fma(-al, ap, rp);
             const double zi = ri * di;
-            x[i] = xi;
+            if (!skip_x) x[i] = xi;
             if (!zr) r[i] = ri;
             pair_t o2;
             o2.x = zi;
@@ -967,7 +981,8 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
             if (i < npad) step(i, f_zp, f_x, f_r, f_ap, f_d);
             i += stride;
         }
-        for (; i < npad; i += stride) step(i, zpv[i], x[i], zr ? 0.0 : r[i], Ap[i], dinv[i]);
+        for (; i < npad; i += stride)
+            step(i, zpv[i], skip_x ? 0.0 : x[i], zr ? 0.0 : r[i], Ap[i], dinv[i]);
     } else {
         const uint32_t nn = npad / 3;
         for (uint32_t v = blockIdx.x * UPD_T + threadIdx.x; v < nn; v += stride) {
@@ -997,6 +1012,24 @@ __global__ __launch_bounds__(UPD_T) void k_cg_update(uint32_t npad, double* __re
     rz = fhd::block_sum<UPD_T>(rz, sh);
     if (threadIdx.x == 0) red_out[blockIdx.x] = make_double2(rr, rz);
     if (lead) st->it_upd = j;
+}
+
+// Deferred x: when the last update (it_upd) left x lagging by alpha p (an odd
+// number of iterations after x_base), add it; p is the pairs' second entry.
+// x_base is then moved by k_x_base, a launch of its own, so no block of this
+// kernel can see it change.
+__global__ __launch_bounds__(256) void k_x_flush(uint32_t npad, double* __restrict__ x,
+                                                 const double* __restrict__ zp,
+                                                 const CgState* st) {
+    const uint64_t j = st->it_upd;
+    if (((j - st->x_base) & 1) == 0) return;
+    const double al = st->alpha[j & 1];
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < npad;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        x[i] = fma(al, zp[2 * i + 1], x[i]);
+}
+__global__ void k_x_base(CgState* st) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->x_base = st->it_upd;
 }
 
 // After a replacement: the next spmv consumes fresh true-residual partials.
@@ -1492,6 +1525,9 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
     // its entries are recovered from z (k_cg_update); the reference's ordered
     // arithmetic and a caller's D^{-1} keep r
     const bool keep_r = PRE != 0 || ORD || args.inv_diag_override;
+    // x updated every second iteration (bitwise the same x): scalar Jacobi,
+    // outside the reference's ordered arithmetic
+    const bool lazy_x = PRE == 0 && !ORD;
     CgCache* cc = nullptr;
     if (!noalloc) {
         const uint64_t key = (uint64_t)BS | (uint64_t)ORD << 4 | (uint64_t)DIFF << 5 |
@@ -1586,8 +1622,17 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
             }
 #undef FH_SPMV
             hipLaunchKernelGGL((k_cg_update<ORD, PRE, BS == 3 && !ORD>), dim3(gu), dim3(UPD_T), 0, s, npad, x,
-                               rvec, w.Ap, w.zp, w.dinv, w.st, w.redA, (int)gc, w.redB);
+                               rvec, w.Ap, w.zp, w.dinv, w.st, w.redA, (int)gc, w.redB,
+                               lazy_x ? 1 : 0);
         }
+    };
+    // complete a deferred x before it is read (replacement, end of the solve)
+    auto flush_x = [&] {
+        if (!lazy_x) return;
+        hipLaunchKernelGGL(k_x_flush, dim3(std::max(1u, std::min<unsigned>(cdiv(npad, 256), 2048))),
+                           dim3(256), 0, s, npad, x, (const double*)w.zp, (const CgState*)w.st);
+        hipLaunchKernelGGL(k_x_base, dim3(1), dim3(64), 0, s, w.st);
+        FH_LAUNCH_CHECK("x_flush");
     };
     uint64_t launched = 0;
     bool first_chunk = true;
@@ -1638,6 +1683,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
                          (unsigned long long)hs->n_restart, (int)lowk, (int)hs->need_rep,
                          (int)hs->stalled);
         if (hs->need_rep) {
+            flush_x();
             // keep the iterate this replacement evaluates (slot = count before k_mark)
             d2d((hs->n_rep & 1) ? w.xs1 : w.xs0, x, npad, s);
             // residual replacement: r = b - A x (double-double evaluation)
@@ -1650,6 +1696,7 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         if (launched > args.max_iter + (hs->n_rep + 2) * (uint64_t)M + 64) break;  // safety net
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    flush_x();
 
     FH_HIP(hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
@@ -1687,9 +1734,11 @@ static int run_cg(const fh_matrix* a, const SolveArgs& args, fh_solve_stats* sta
         stats->spmv32_bytes = diff_spmv_bytes(a, 4);
         // SpMV + the vector passes of the fused loop: the spmv kernel gathers
         // (z, p_old) pairs (one vector beyond the SpMV's x) and writes Ap, the
-        // update reads x, r, Ap, the pairs and D^{-1} and writes x, r, the pairs
+        // update reads Ap, the pairs and D^{-1}, reads and writes the pairs, x
+        // (every second iteration when deferred) and r (when kept)
         const double dinv_b = PRE == 0 ? 8.0 * n : 24.0 * n;
-        stats->iter_bytes = stats->spmv_bytes + 8.0 * n * (1.0 + 9.0) + dinv_b;
+        const double upd_vectors = 1.0 + 4.0 + (lazy_x ? 1.0 : 2.0) + (keep_r ? 2.0 : 0.0);
+        stats->iter_bytes = stats->spmv_bytes + 8.0 * n * (1.0 + upd_vectors) + dinv_b;
         const double it = (double)std::max<uint64_t>(stats->iterations, 1);
         const double tms = std::max(stats->kernel_ms, 1e-9);
         stats->effective_bw_gbps = stats->iter_bytes * it / (tms * 1e-3) / 1e9;

@@ -282,6 +282,22 @@ def test_solve_without_stored_residual_matches_kept_residual():
         assert np.linalg.norm(b - S @ r.solution) <= 2e-10 * np.linalg.norm(b)
 
 
+def test_deferred_x_after_each_iteration_count():
+    """The update kernel writes x every second iteration and a flush completes
+    it before it is read.  Stopped after k = 1..6 iterations (both parities),
+    the GPU iterate equals the oracle's cg.rs iterate after k iterations to
+    rounding; a missing flush (x short of alpha_k p_k) or a doubled one fails
+    by far more."""
+    Ab, bb = c0_system()
+    K = fh.matrix_from_csc(*Ab.export())
+    for k in range(1, 7):
+        st, xo, it, _ = O.solve_cg(Ab, bb, None, 1e-30, k)
+        assert st == 2 and it == k
+        x, s = K.solve(bb, tol=1e-30, max_iter=k, precision=fh.FP64, allow_fail=True)
+        assert s.iterations == k
+        assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo), k
+
+
 def bsr3_synthetic(nx=15):
     """gpu.rs:1671-1741: synthetic SPD 3x3 block-CSR on a 15^3 grid."""
     n_nodes = nx ** 3
