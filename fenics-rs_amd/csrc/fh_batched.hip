@@ -245,13 +245,14 @@ struct BbState {
 constexpr int NA = NSYM + NR * NR;  // W^T A W (packed, from the product), W^T R (from W = Z + P beta)
 constexpr int NB = NR * NR + NR;    // update partials: Q^T Z, R^T R (diagonal)
 
-// R = B (X0 = 0), Z = D^{-1} R (the (z, p) row pairs zp = [z0..z3, p0..p3],
-// p = 0); tile partials R^T R and B^T B (diagonals) in the first 2 NR entries
-// of an NB record
+// R = B (X0 = 0), Z = D^{-1} R (FP64 rows of four), P = 0 (rows of four in
+// the storage type TV); tile partials R^T R and B^T B (diagonals) in the
+// first 2 NR entries of an NB record.  R itself is not stored: the kernels
+// that need it take R = Z / D^{-1} (D^{-1} has no zero entry)
 template <class TV>
 __global__ __launch_bounds__(BT) void k_bb_init(const double* __restrict__ b,
                                                 const double* __restrict__ dinv,
-                                                double* __restrict__ r, TV* __restrict__ zp,
+                                                double* __restrict__ z, TV* __restrict__ p,
                                                 double* __restrict__ partB, int s) {
     __shared__ double sh[4 * 2 * NR];
     const uint32_t t = blockIdx.x;
@@ -270,9 +271,8 @@ __global__ __launch_bounds__(BT) void k_bb_init(const double* __restrict__ b,
             zv[k] = bv[k] * di;
             acc[k] += bv[k] * bv[k];
         }
-        st4(r + row * NR, bv);
-        st4(zp + row * 2 * NR, zv);
-        st4(zp + row * 2 * NR + NR, zero);
+        st4(z + row * NR, zv);
+        st4(p + row * NR, zero);
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[NR + k] = acc[k];
@@ -307,10 +307,12 @@ __global__ __launch_bounds__(BT) void k_bb_start(BbState S, const double* __rest
 }
 
 // W = Z + P beta (rounded to the storage type TV) for the rows of the active
-// tiles; tile partials W^T R into entries NSYM.. of the tile's NA record
+// tiles; tile partials W^T R (R = Z / D^{-1}) into entries NSYM.. of the
+// tile's NA record
 template <class TV>
-__global__ __launch_bounds__(BT) void k_bb_formw(const TV* __restrict__ zp,
-                                                 const double* __restrict__ r,
+__global__ __launch_bounds__(BT) void k_bb_formw(const double* __restrict__ z,
+                                                 const TV* __restrict__ p,
+                                                 const double* __restrict__ dinv,
                                                  TV* __restrict__ w, BbState S,
                                                  double* __restrict__ partA) {
     __shared__ double sh[4 * NR * NR];
@@ -330,9 +332,11 @@ __global__ __launch_bounds__(BT) void k_bb_formw(const TV* __restrict__ zp,
         for (int sub = 0; sub < TR / BT; ++sub) {
             const uint64_t row = (uint64_t)t * TR + sub * BT + threadIdx.x;
             double zo[NR], po[NR], wo[NR], rv[NR];
-            ld4(zp + row * 2 * NR, zo);
-            ld4(zp + row * 2 * NR + NR, po);
-            ld4(r + row * NR, rv);
+            ld4(z + row * NR, zo);
+            ld4(p + row * NR, po);
+            const double di = dinv[row];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) rv[k] = zo[k] / di;
             form_w(zo, po, beta, act, wo);
 #pragma unroll
             for (int k = 0; k < NR; ++k) wo[k] = rnd<TV>(wo[k]);
@@ -448,14 +452,15 @@ __global__ __launch_bounds__(BT) void k_bb_red1(BbState S, const double* __restr
 
 // P = W L^{-T}, Q = Q_W L^{-T} (FP64, from the stored W and Q_W = A W, so X
 // and R move by a consistent pair and the recursive residual tracks b - A x
-// to FP64 rounding whatever TV is); X += P alpha, R -= Q alpha, Z = D^{-1} R;
-// the (z, p) pair stored as TV; tile partials Q^T Z (the stored z) and R^T R
+// to FP64 rounding whatever TV is); R = Z / D^{-1} (R is not stored: 64
+// bytes per row less to stream), X += P alpha, R -= Q alpha, Z = D^{-1} R;
+// Z stored in FP64, P as TV; tile partials Q^T Z and R^T R
 template <class TV>
 __global__ __launch_bounds__(BT) void k_bb_upd(const TV* __restrict__ w,
                                                const double* __restrict__ qw,
                                                const double* __restrict__ dinv,
-                                               double* __restrict__ x, double* __restrict__ r,
-                                               TV* __restrict__ zp, BbState S,
+                                               double* __restrict__ x, double* __restrict__ z,
+                                               TV* __restrict__ p, BbState S,
                                                double* __restrict__ partB) {
     __shared__ double sh[4 * NB];
     const uint32_t na = S.nact[0];
@@ -480,10 +485,12 @@ __global__ __launch_bounds__(BT) void k_bb_upd(const TV* __restrict__ w,
         ld4(w + row * NR, wv4);
         ld4(qw + row * NR, qw4);
         ld4(x + row * NR, xv);
-        ld4(r + row * NR, rv);
+        ld4(z + row * NR, zv);
         lsolve_row(L, kept, wv4, pv);
         lsolve_row(L, kept, qw4, qv);
         const double di = dinv[row];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) rv[k] = zv[k] / di;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             double dx = 0.0, dr = 0.0;
@@ -494,12 +501,11 @@ __global__ __launch_bounds__(BT) void k_bb_upd(const TV* __restrict__ w,
             }
             xv[k] += dx;
             rv[k] -= dr;
-            zv[k] = rnd<TV>(rv[k] * di);
+            zv[k] = rv[k] * di;
         }
         st4(x + row * NR, xv);
-        st4(r + row * NR, rv);
-        st4(zp + row * 2 * NR, zv);
-        st4(zp + row * 2 * NR + NR, pv);
+        st4(z + row * NR, zv);
+        st4(p + row * NR, pv);
 #pragma unroll
         for (int a2 = 0; a2 < NR; ++a2)
 #pragma unroll
@@ -838,19 +844,19 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
     DBuf<uint8_t> status(dev, n_inst * NR);
     DBuf<double> bnorm(dev, n_inst * NR), beta(dev, n_inst * NR * NR), Lm(dev, n_inst * NR * NR),
         alpha(dev, n_inst * NR * NR);
-    // Krylov vectors: x, r and q = A w in FP64; the gathered (z, p) pairs and w
-    // in TV (FP32 in the mixed modes)
+    // Krylov vectors: x, z and q = A w in FP64 (r is not stored: R = Z / D^{-1});
+    // p and the gathered w in TV (FP32 in the mixed modes)
     using TV = T;
-    DBuf<double> x(dev, nvec), r(dev, nvec), q(dev, nvec), partA(dev, (size_t)n_tiles * NA),
+    DBuf<double> x(dev, nvec), z(dev, nvec), q(dev, nvec), partA(dev, (size_t)n_tiles * NA),
         partB(dev, (size_t)n_tiles * NB), part_t(dev, (size_t)n_tiles * NR);
-    DBuf<TV> zp(dev, 2 * nvec), wb(dev, nvec);
+    DBuf<TV> pb(dev, nvec), wb(dev, nvec);
     // padding rows and the rows of the unused columns stay zero in every vector
-    for (double* v : {x.p, r.p, q.p}) fill_f64(v, nvec, 0.0, s);
+    for (double* v : {x.p, z.p, q.p}) fill_f64(v, nvec, 0.0, s);
     if constexpr (MIX) {
-        fill_f32((float*)zp.p, 2 * nvec, 0.0f, s);
+        fill_f32((float*)pb.p, nvec, 0.0f, s);
         fill_f32((float*)wb.p, nvec, 0.0f, s);
     } else {
-        fill_f64((double*)zp.p, 2 * nvec, 0.0, s);
+        fill_f64((double*)pb.p, nvec, 0.0, s);
         fill_f64((double*)wb.p, nvec, 0.0, s);
     }
     BbState S{act.p, kept.p, itc.p, done.p, bnorm.p, beta.p, Lm.p, alpha.p, iters.p, status.p,
@@ -867,7 +873,7 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
     FH_HIP(hipStreamSynchronize(s));
 
     Timer tk(s);
-    hipLaunchKernelGGL(k_bb_init<TV>, dim3(n_tiles), dim3(BT), 0, s, b.p, dinv.p, r.p, zp.p,
+    hipLaunchKernelGGL(k_bb_init<TV>, dim3(n_tiles), dim3(BT), 0, s, b.p, dinv.p, z.p, pb.p,
                        partB.p, (int)nrhs);
     hipLaunchKernelGGL(k_bb_start, dim3((unsigned)n_inst), dim3(BT), 0, s, S, partB.p);
     hipLaunchKernelGGL(k_bb_compact, dim3(1), dim3(1024), 0, s, S);
@@ -875,13 +881,13 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
     const int Mch = tn.chunk;
     auto chunk = [&]() {
         for (int m = 0; m < Mch; ++m) {
-            hipLaunchKernelGGL(k_bb_formw<TV>, dim3(gt), dim3(BT), 0, s, (const TV*)zp.p,
-                               (const double*)r.p, wb.p, S, partA.p);
+            hipLaunchKernelGGL(k_bb_formw<TV>, dim3(gt), dim3(BT), 0, s, (const double*)z.p,
+                               (const TV*)pb.p, (const double*)dinv.p, wb.p, S, partA.p);
             hipLaunchKernelGGL((k_bb_prod<T, TV>), dim3(gt), dim3(BT), 0, s, AK, (const TV*)wb.p,
                                q.p, S, partA.p);
             hipLaunchKernelGGL(k_bb_red1, dim3(gi), dim3(BT), 0, s, S, partA.p);
             hipLaunchKernelGGL(k_bb_upd<TV>, dim3(gt), dim3(BT), 0, s, (const TV*)wb.p,
-                               (const double*)q.p, (const double*)dinv.p, x.p, r.p, zp.p, S,
+                               (const double*)q.p, (const double*)dinv.p, x.p, z.p, pb.p, S,
                                partB.p);
             hipLaunchKernelGGL(k_bb_red2, dim3(gi), dim3(BT), 0, s, S, partB.p);
         }
