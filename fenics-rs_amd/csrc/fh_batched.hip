@@ -996,16 +996,16 @@ static int run_batched(uint64_t n_inst, const uint64_t* sizes, uint64_t nrhs,
         // DOF (unknowns x columns) * iterations per second
         stats->gflops = dof_iters / (kms * 1e-3) / 1e9;
         // one product pass: the matrix once for all columns, one read of the
-        // gathered w, the q store; vector rows are 4 columns of 8 bytes (x, r,
-        // q) or of sizeof(T) (z, p, w)
+        // gathered w, the q store; vector rows are 4 columns of 8 bytes (x, z,
+        // q) or of sizeof(T) (p, w)
         const double vb = 4.0 * sizeof(T);
         stats->spmv_bytes = (double)Ac->pat->nnzb * (sizeof(T) + 4.0) + 4.0 * (V + 1) +
                             (vb + 32.0) * (double)V;
-        // one block iteration: W = Z + P beta (reads the (z, p) pair and r,
-        // writes w), the product pass, and the update (reads w, q, x, r and
-        // D^{-1}; writes x, r and the (z, p) pair)
-        stats->iter_bytes = stats->spmv_bytes + (3.0 * vb + 32.0) * (double)V +
-                            (vb + 32.0 * 3.0 + 8.0 + 32.0 * 2.0 + 2.0 * vb) * (double)V;
+        // one block iteration: W = Z + P beta (reads z, p and D^{-1}, writes
+        // w), the product pass, and the update (reads w, q, x, z and D^{-1};
+        // writes x, z and p)
+        stats->iter_bytes = stats->spmv_bytes + (2.0 * vb + 40.0) * (double)V +
+                            (2.0 * vb + 32.0 * 3.0 + 8.0 + 32.0 * 2.0) * (double)V;
         stats->effective_bw_gbps = 0.0;
         if (kms > 0.0) {
             double bytes = 0.0;
