@@ -244,7 +244,7 @@ def main():
                          "copy (hardware-counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the benchmark runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is designed but not a measured path.",
               file=sys.stderr)
         sys.exit(2)
