@@ -332,6 +332,17 @@ def test_unstructured_mesh_bitwise(dim):
         Kco, bco = O.apply_dirichlet_bc(Ko, bo, [d for d, _ in fixed], [v for _, v in fixed])
         same_csc(Kc, Kco)
         assert np.array_equal(bits(bc), bits(bco))
+        # the P2 numbering (cells around each edge, then max - min, then min:
+        # on a renumbered mesh max - min is no direction, only the tie rule)
+        # and the P2 stiffness, bitwise; the MFMA elasticity within its bound
+        p2 = O.P2(om)
+        assert gm.p2_info() == (p2.ndofs, p2.nedges)
+        assert np.array_equal(gm.p2_cell_dofs(), p2.cell_dofs())
+        same_csc(fh.assemble_p2_stiffness(gm, 0), p2.stiffness())
+        _, _, v64 = K.export_csc()
+        _, _, v32 = fh.assemble_elasticity_stiffness_3d(gm, fh.ElasticMaterial(7e10, 0.33),
+                                                        method=1).export_csc()
+        assert np.abs(v32 - v64).max() <= 2e-6 * np.abs(v64).max()
 
 
 def test_high_valence_vertex_bitwise():
